@@ -1,0 +1,115 @@
+/*
+ * wsr.h -- C ABI of the MI355X WENO5-SR finite-volume library (libwsr_hip.so).
+ *
+ * All state pointers are DEVICE pointers (owned by the caller, e.g. torch
+ * tensors) in the reference layout: FP64, C-order (ny + 2*ghost, nx + 2*ghost, 4),
+ * conserved variables (rho, rho*u, rho*v, E), ghost = 3.  Every call is
+ * stream-ordered on `stream` (a hipStream_t; NULL = default stream) and returns
+ * 0 on success or a negative WSR_E* code; nothing throws across the ABI.  The
+ * library keeps no global mutable state: per-run state lives in the caller's
+ * workspace buffer.
+ *
+ * Reference paths are relative to the reference checkout.
+ */
+#ifndef WSR_H
+#define WSR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reconstruction closures */
+enum {
+    WSR_SCHEME_JS = 0,     /* WENO5-JS (code/snapshot/warp_weno5_helpers.py:511-570) */
+    WSR_SCHEME_SR64 = 1,   /* WENO5-SR, FP64 MLP (teacherfree_lab_weno5_v20_distance_balanced/weno5_rk3_diff_v20_deploy.py:155-281) */
+    WSR_SCHEME_SR32 = 2,   /* WENO5-SR, FP32 MLP (teacherfree_lab_weno5_v20_distance_balanced_mlp_f32_fast/warp_mlp_f32/weno5_rk3_diff_mlp_f32.py:152-283) */
+    WSR_SCHEME_SRBF16 = 3, /* WENO5-SR, BF16 MLP parameters/activations (no reference counterpart) */
+    WSR_SCHEME_Z = 4       /* WENO5-Z p=2 (code/snapshot/weno_z_borges_p2_results/weno5_helpers_z.py:511-570) */
+};
+/* Riemann fluxes */
+enum {
+    WSR_RIEMANN_HLLC = 0,  /* code/snapshot/for_paper_results/solvers/weno5_hllc.py:23-115 */
+    WSR_RIEMANN_FORCE = 1  /* code/snapshot/warp_weno5_helpers.py:401-435 */
+};
+/* boundary conditions */
+enum {
+    WSR_BC_TRANSMISSIVE = 0, /* weno5_rk3_diff_v20_deploy.py:391-409 */
+    WSR_BC_PERIODIC = 1,     /* weno5_rk3_diff_v20_deploy.py:371-388 */
+    WSR_BC_DOUBLE_MACH = 2   /* teacherfree_lab_weno5_v20_distance_balanced/weno5_hllc_refsym.py:25-124 */
+};
+/* error codes */
+enum {
+    WSR_OK = 0,
+    WSR_E_ARG = -1,        /* bad size / enum / null pointer */
+    WSR_E_MLP = -2,        /* SR scheme without MLP parameters */
+    WSR_E_WORKSPACE = -3,  /* workspace too small */
+    WSR_E_HIP = -4         /* a HIP runtime call failed */
+};
+
+/* Grid and method description (a plain C struct; mirrors the fields of
+ * Params, code/snapshot/warp_weno5_helpers.py:66-87, plus the method switches). */
+typedef struct wsr_euler2d_config {
+    int32_t nx, ny, ghost;          /* ghost must be 3 */
+    int32_t scheme, riemann, boundary;
+    int32_t eno_cutoff;             /* 0/1, SR only (weno5_rk3_diff_v20_deploy.py:258-279) */
+    int32_t reserved;
+    double dx, dy, gamma;
+    double cfl, t_end;              /* used by the device-resident time loop only */
+    /* host pointer to the 189 closure parameters w1,b1,w2,b2,w3,b3,w4,b4
+     * (row-major, no singleton axis): double for SR64, float for SR32/SRBF16;
+     * NULL for JS and Z.  Copied into the kernel arguments at every launch. */
+    const void *mlp;
+} wsr_euler2d_config;
+
+/* Bytes of device workspace the calls below need (fluxes, boundary-filled
+ * stage copy, device time-loop control block). */
+size_t wsr_euler2d_workspace_bytes(int32_t nx, int32_t ny, int32_t ghost);
+
+/* One SSPRK3 stage: boundary fill, x and y sweeps (characteristic normal
+ * reconstruction, transverse two-point Gauss reconstruction, Riemann flux) and
+ * the stage combination u_out = {rhs, 3/4 u0 + 1/4 rhs, 1/3 u0 + 2/3 rhs}.
+ * Replaces launch_stage of for_paper_results/solvers/weno5_hllc_mixed.py:121-168,
+ * teacherfree_lab_weno5_v20_distance_balanced/weno5_hllc_refsym.py:298-435 and
+ * for_paper_results/solvers/weno5_hllc.py:390-454.  Only the interior of u_out
+ * is written.  u_stage == u0 is allowed (stage 1); u_out may alias u0. */
+int wsr_euler2d_stage(const wsr_euler2d_config *cfg, const double *u_stage, const double *u0,
+                      double *u_out, double dt, int32_t rk, double boundary_time,
+                      void *workspace, size_t workspace_bytes, void *stream);
+
+/* Maximum characteristic speed max(|u|+c, |v|+c) over the interior, reduced on
+ * the device and returned to the host (synchronises the stream).  Replaces
+ * compute_max_speed_kernel + np.max of warp_weno5_helpers.py:189-207,876-897;
+ * NaN anywhere gives NaN like np.max. */
+int wsr_euler2d_max_speed(const wsr_euler2d_config *cfg, const double *u, void *workspace,
+                          size_t workspace_bytes, double *max_speed_out, void *stream);
+
+/* Device-resident time loop (the `while t < t_end` loop of run_to_time,
+ * weno5_hllc_mixed.py:185-212 / weno5_hllc_refsym.py:462-544 /
+ * weno5_hllc.py:482-506) without a host round trip per step.
+ *   loop_init : t = 0, step = 0, first max speed of u (enqueued, no sync).
+ *   loop_steps: enqueue n steps; each step computes dt = min(cfl*min(dx,dy)/max_speed,
+ *               t_end - t) on the device, stops the loop (later steps become no-ops)
+ *               when t >= t_end - 1e-14, dt is non-finite or <= 0, or max_steps is
+ *               reached, runs the three stages into u1, u2 and back into u, records
+ *               dts[step] (device array of max_steps doubles) and folds the next
+ *               max speed into the third stage.
+ *   loop_status: synchronise and read step count, time, last dt and the done flag. */
+int wsr_euler2d_loop_init(const wsr_euler2d_config *cfg, const double *u, int32_t max_steps,
+                          void *workspace, size_t workspace_bytes, void *stream);
+int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1, double *u2,
+                           double *dts, int32_t n_steps, void *workspace, size_t workspace_bytes,
+                           void *stream);
+int wsr_euler2d_loop_status(void *workspace, int32_t *steps, double *t, double *dt_last,
+                            int32_t *done, void *stream);
+
+/* Library version / build identification. */
+const char *wsr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WSR_H */

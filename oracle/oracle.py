@@ -1,0 +1,241 @@
+"""ctypes front-end of the C oracle (``wsr_oracle.c``) -- test infrastructure only.
+
+The oracle restates the reference's FP64 two-dimensional WENO5 step in plain C
+(see the citations inside ``wsr_oracle.c``).  Arrays are numpy float64 in the
+reference layout ``(ny + 2*ghost, nx + 2*ghost, 4)``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "libwsr_oracle.so"
+
+SCHEMES = {"js": 0, "sr64": 1, "sr32": 2, "srbf16": 3, "z": 4}
+RIEMANN = {"hllc": 0, "force": 1}
+BOUNDARY = {"transmissive": 0, "periodic": 1, "double-mach": 2}
+HEADS = {"normal_lr1": 0, "normal_lr2": 1, "gauss_lr1": 2, "gauss_lr2": 3}
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_P = ctypes.c_void_p
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile the oracle with its Makefile (no FMA contraction, no fast-math)."""
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "wsr_oracle.c").stat().st_mtime:
+        subprocess.run(["make", "-s", "-C", str(HERE), "-B" if force else "all"], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        L = ctypes.CDLL(str(LIB_PATH))
+        L.oracle_euler2d_work_doubles.restype = ctypes.c_size_t
+        L.oracle_euler2d_work_doubles.argtypes = [_I, _I, _I]
+        L.oracle_euler2d_stage.restype = _I
+        L.oracle_euler2d_stage.argtypes = [_P, _P, _P, _I, _I, _I, _D, _D, _D, _D, _I, _I, _I, _I, _D, _I, _P, _P]
+        L.oracle_euler2d_max_speed.restype = _D
+        L.oracle_euler2d_max_speed.argtypes = [_P, _I, _I, _I, _D]
+        L.oracle_euler2d_dt.restype = _D
+        L.oracle_euler2d_dt.argtypes = [_P, _I, _I, _I, _D, _D, _D, _D]
+        L.oracle_euler2d_run.restype = _I
+        L.oracle_euler2d_run.argtypes = [_P, _I, _I, _I, _D, _D, _D, _D, _D, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]
+        for name in ("oracle_hllc",):
+            getattr(L, name).argtypes = [_P, _P, _I, _D, _P]
+        L.oracle_force.argtypes = [_P, _P, _I, _D, _D, _P]
+        L.oracle_physical_flux.argtypes = [_P, _I, _D, _P]
+        L.oracle_roe.argtypes = [_P, _P, _D, _P]
+        L.oracle_char_roundtrip.argtypes = [_P, _P, _I, _D, _P, _P]
+        L.oracle_candidates.argtypes = [_P, _I, _P]
+        L.oracle_scalar_value.restype = _D
+        L.oracle_scalar_value.argtypes = [_P, _I, _I, _P, _I, _D]
+        L.oracle_features.argtypes = [_P, _P]
+        L.oracle_plateau.restype = _I
+        L.oracle_plateau.argtypes = [_P]
+        L.oracle_mlp_raw.argtypes = [_P, _I, _I, _P, _P]
+        L.oracle_mlp_weights.argtypes = [_P, _I, _I, _P, _I, _P]
+        L.oracle_round_bf16.restype = ctypes.c_float
+        L.oracle_round_bf16.argtypes = [ctypes.c_float]
+        L.oracle_num_threads.restype = _I
+        L.oracle_set_num_threads.argtypes = [_I]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def pack_mlp(mlp: dict | None, scheme: str) -> np.ndarray | None:
+    """Flatten w1,b1,...,b4 (leading singleton axis allowed) in the oracle layout."""
+    if mlp is None or scheme in ("js", "z"):
+        return None
+    dtype = np.float64 if scheme == "sr64" else np.float32
+    parts = [np.asarray(mlp[name]).reshape(-1) for name in ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4")]
+    flat = np.concatenate(parts).astype(dtype)
+    if flat.size != 189:
+        raise ValueError(f"WENO5 MLP must have 189 parameters, got {flat.size}")
+    return np.ascontiguousarray(flat)
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_num_threads(int(n))
+
+
+def num_threads() -> int:
+    return int(lib().oracle_num_threads())
+
+
+def stage2d(u_stage, u0, nx, ny, dx, dy, gamma, dt, rk, scheme="js", riemann="hllc",
+            boundary="transmissive", boundary_time=0.0, eno_cutoff=False, mlp=None, ghost=3):
+    u_stage = _f64(u_stage)
+    u0 = _f64(u0)
+    out = np.zeros_like(u_stage)
+    packed = pack_mlp(mlp, scheme)
+    work = np.zeros(lib().oracle_euler2d_work_doubles(nx, ny, ghost), dtype=np.float64)
+    rc = lib().oracle_euler2d_stage(
+        _ptr(u_stage), _ptr(u0), _ptr(out), nx, ny, ghost, dx, dy, gamma, dt, rk,
+        SCHEMES[scheme], RIEMANN[riemann], BOUNDARY[boundary], boundary_time,
+        1 if eno_cutoff else 0, _ptr(packed), _ptr(work))
+    if rc:
+        raise RuntimeError(f"oracle_euler2d_stage failed: {rc}")
+    return out
+
+
+def max_speed2d(u, nx, ny, gamma, ghost=3) -> float:
+    u = _f64(u)
+    return float(lib().oracle_euler2d_max_speed(_ptr(u), nx, ny, ghost, gamma))
+
+
+def dt2d(u, nx, ny, dx, dy, gamma, cfl, ghost=3) -> float:
+    u = _f64(u)
+    return float(lib().oracle_euler2d_dt(_ptr(u), nx, ny, ghost, dx, dy, gamma, cfl))
+
+
+def step2d(u, nx, ny, dx, dy, gamma, dt, t=0.0, **kw):
+    """One SSPRK3 step (three stages) following A64:486-526."""
+    u1 = stage2d(u, u, nx, ny, dx, dy, gamma, dt, 1, boundary_time=t, **kw)
+    u2 = stage2d(u1, u, nx, ny, dx, dy, gamma, dt, 2, boundary_time=t + dt, **kw)
+    u3 = stage2d(u2, u, nx, ny, dx, dy, gamma, dt, 3, boundary_time=t + 0.5 * dt, **kw)
+    return u3
+
+
+def run2d(u, nx, ny, dx, dy, gamma, cfl, t_end, scheme="js", riemann="hllc",
+          boundary="transmissive", eno_cutoff=False, mlp=None, max_steps=1_000_000, ghost=3):
+    """run_to_time without reporting: returns (state, dts, steps, t)."""
+    u = _f64(u).copy()
+    n = u.size
+    tmp = np.zeros(3 * n, dtype=np.float64)
+    work = np.zeros(lib().oracle_euler2d_work_doubles(nx, ny, ghost), dtype=np.float64)
+    cap = max(1, min(max_steps, 1_000_000))
+    dts = np.zeros(cap, dtype=np.float64)
+    t = ctypes.c_double(0.0)
+    packed = pack_mlp(mlp, scheme)
+    steps = lib().oracle_euler2d_run(
+        _ptr(u), nx, ny, ghost, dx, dy, gamma, cfl, t_end, cap, SCHEMES[scheme],
+        RIEMANN[riemann], BOUNDARY[boundary], 1 if eno_cutoff else 0, _ptr(packed),
+        _ptr(tmp), _ptr(work), _ptr(dts), ctypes.addressof(t))
+    if steps < 0:
+        raise RuntimeError(f"oracle_euler2d_run failed: {steps}")
+    return u, dts[:steps].tolist(), int(steps), float(t.value)
+
+
+# ---------------------------------------------------------------- unit level
+def hllc(ul, ur, direction, gamma=1.4):
+    ul, ur = _f64(ul), _f64(ur)
+    out = np.zeros(4)
+    lib().oracle_hllc(_ptr(ul), _ptr(ur), direction, gamma, _ptr(out))
+    return out
+
+
+def force(ul, ur, direction, c, gamma=1.4):
+    ul, ur = _f64(ul), _f64(ur)
+    out = np.zeros(4)
+    lib().oracle_force(_ptr(ul), _ptr(ur), direction, c, gamma, _ptr(out))
+    return out
+
+
+def physical_flux(q, direction, gamma=1.4):
+    q = _f64(q)
+    out = np.zeros(4)
+    lib().oracle_physical_flux(_ptr(q), direction, gamma, _ptr(out))
+    return out
+
+
+def roe(left, right, gamma=1.4):
+    left, right = _f64(left), _f64(right)
+    out = np.zeros(4)
+    lib().oracle_roe(_ptr(left), _ptr(right), gamma, _ptr(out))
+    return out
+
+
+def char_roundtrip(q, roe_state, direction, gamma=1.4):
+    q, roe_state = _f64(q), _f64(roe_state)
+    c = np.zeros(4)
+    back = np.zeros(4)
+    lib().oracle_char_roundtrip(_ptr(q), _ptr(roe_state), direction, gamma, _ptr(c), _ptr(back))
+    return c, back
+
+
+def candidates(q, head):
+    q = _f64(q)
+    s = np.zeros(3)
+    lib().oracle_candidates(_ptr(q), HEADS[head], _ptr(s))
+    return s
+
+
+def scalar_value(q, head, scheme="js", mlp=None, eno_cutoff=False, h=0.0):
+    q = _f64(q)
+    packed = pack_mlp(mlp, scheme)
+    return float(lib().oracle_scalar_value(_ptr(q), HEADS[head], SCHEMES[scheme], _ptr(packed),
+                                           1 if eno_cutoff else 0, h))
+
+
+def features(q):
+    q = _f64(q)
+    f = np.zeros(5)
+    lib().oracle_features(_ptr(q), _ptr(f))
+    return f
+
+
+def plateau(q) -> bool:
+    q = _f64(q)
+    return bool(lib().oracle_plateau(_ptr(q)))
+
+
+def mlp_raw(feats, scheme, mlp):
+    feats = _f64(np.atleast_2d(feats))
+    out = np.zeros((feats.shape[0], 3))
+    packed = pack_mlp(mlp, scheme)
+    lib().oracle_mlp_raw(_ptr(feats), feats.shape[0], SCHEMES[scheme], _ptr(packed), _ptr(out))
+    return out
+
+
+def mlp_weights(q, head, scheme, mlp, eno_cutoff=False):
+    q = _f64(q)
+    w = np.zeros(3)
+    packed = pack_mlp(mlp, scheme)
+    lib().oracle_mlp_weights(_ptr(q), HEADS[head], SCHEMES[scheme], _ptr(packed),
+                             1 if eno_cutoff else 0, _ptr(w))
+    return w
+
+
+def round_bf16(x: float) -> float:
+    return float(lib().oracle_round_bf16(float(x)))
+
+
+os.environ.setdefault("OMP_PROC_BIND", "false")

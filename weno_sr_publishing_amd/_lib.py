@@ -1,0 +1,74 @@
+"""ctypes binding of ``libwsr_hip.so`` (the C ABI declared in include/wsr.h).
+
+The HIP library is the only compute path: if it is missing this module raises,
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libwsr_hip.so"
+
+SCHEMES = {"js": 0, "sr64": 1, "sr32": 2, "srbf16": 3, "z": 4}
+RIEMANN = {"hllc": 0, "force": 1}
+BOUNDARY = {"transmissive": 0, "periodic": 1, "double-mach": 2}
+ERRORS = {-1: "bad argument", -2: "SR scheme without MLP parameters", -3: "workspace too small",
+          -4: "HIP runtime error"}
+
+EXPORTS = (
+    "wsr_euler2d_workspace_bytes", "wsr_euler2d_stage", "wsr_euler2d_max_speed",
+    "wsr_euler2d_loop_init", "wsr_euler2d_loop_steps", "wsr_euler2d_loop_status", "wsr_version",
+)
+
+
+class Euler2DConfig(ctypes.Structure):
+    """Mirror of ``wsr_euler2d_config``."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ghost", ctypes.c_int32),
+        ("scheme", ctypes.c_int32), ("riemann", ctypes.c_int32), ("boundary", ctypes.c_int32),
+        ("eno_cutoff", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("mlp", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+
+class WsrError(RuntimeError):
+    pass
+
+
+def check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise WsrError(f"{what} failed: {ERRORS.get(rc, rc)} ({rc})")
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is not built; run `python -m weno_sr_publishing_amd.build` "
+                "(the HIP library is the only compute path)")
+        L = ctypes.CDLL(str(LIB_PATH))
+        P, I32, D, SZ = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double, ctypes.c_size_t
+        CFG = ctypes.POINTER(Euler2DConfig)
+        L.wsr_euler2d_workspace_bytes.restype = SZ
+        L.wsr_euler2d_workspace_bytes.argtypes = [I32, I32, I32]
+        L.wsr_euler2d_stage.restype = ctypes.c_int
+        L.wsr_euler2d_stage.argtypes = [CFG, P, P, P, D, I32, D, P, SZ, P]
+        L.wsr_euler2d_max_speed.restype = ctypes.c_int
+        L.wsr_euler2d_max_speed.argtypes = [CFG, P, P, SZ, ctypes.POINTER(D), P]
+        L.wsr_euler2d_loop_init.restype = ctypes.c_int
+        L.wsr_euler2d_loop_init.argtypes = [CFG, P, I32, P, SZ, P]
+        L.wsr_euler2d_loop_steps.restype = ctypes.c_int
+        L.wsr_euler2d_loop_steps.argtypes = [CFG, P, P, P, P, I32, P, SZ, P]
+        L.wsr_euler2d_loop_status.restype = ctypes.c_int
+        L.wsr_euler2d_loop_status.argtypes = [P, ctypes.POINTER(I32), ctypes.POINTER(D),
+                                              ctypes.POINTER(D), ctypes.POINTER(I32), P]
+        L.wsr_version.restype = ctypes.c_char_p
+        _lib = L
+    return _lib

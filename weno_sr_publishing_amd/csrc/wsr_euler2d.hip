@@ -1,0 +1,566 @@
+// wsr_euler2d.hip -- MI355X (gfx950) kernels and C ABI of the 2-D WENO5 step.
+//
+// One SSPRK3 stage is four launches:
+//   k_boundary : ghost fill of the stage state into the workspace copy `bc`
+//                (transmissive / periodic / double-Mach).
+//   k_sweep<X> : fused x sweep.  Phase 1 computes, per x-face and row, the Roe
+//                average, the characteristic projection of the six stencil
+//                cells, the two WENO5 face values (left and right state of the
+//                face) and their back-projection, into LDS.  Phase 2 computes,
+//                per x-face, the transverse two-point Gauss reconstruction of
+//                the eight face-value components (closure shared by both Gauss
+//                points) and the two Riemann fluxes, and writes
+//                Fx = (F(g1) + F(g2)) * dt/dx.
+//   k_sweep<Y> : the same along y -> Fy.
+//   k_update   : u_out = SSPRK3 combination of u0 and u - (dFx + dFy)/2; in the
+//                device time loop it also reduces the next step's max speed.
+// Each face is owned by eight lanes (side s = 0/1 x component k = 0..3); the
+// lanes of a face exchange values with cross-lane shuffles.  The MLP weights
+// travel in the kernel arguments (189 FP32/FP64 values), so they are read
+// with scalar loads and shared by the whole wavefront.
+//
+// Reference (paths relative to the reference checkout):
+//   launch order / stage: for_paper_results/solvers/weno5_hllc_mixed.py:121-168
+//   normal reconstruction: weno5_rk3_diff_v20_deploy.py:480-596, warp_weno5_helpers.py:948-1078
+//   transverse + flux: for_paper_results/solvers/weno5_hllc_mixed.py:21-108, weno5_hllc.py:226-285
+//   update: weno5_rk3_diff.py:1003-1032; boundary: weno5_rk3_diff_v20_deploy.py:371-409,
+//   teacherfree_lab_weno5_v20_distance_balanced/weno5_hllc_refsym.py:25-124
+//   dt: warp_weno5_helpers.py:189-207,876-897; loop: weno5_hllc_mixed.py:185-212
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../../include/wsr.h"
+#include "wsr_math.hpp"
+
+namespace wsr {
+
+constexpr int BLOCK = 256;
+constexpr int TN = 8;    // faces per tile along the sweep axis
+constexpr int TT = 28;   // transverse lines per tile (phase 1 covers TT + 4)
+constexpr int NT1 = TT + 4;
+constexpr int NN = TN + 5;  // stencil cells per transverse line in a tile
+
+struct Ctl {
+    double t, dt, t_end, cfl_h, dt_last;
+    unsigned long long speed_bits;
+    int32_t step, done, nan_flag, max_steps;
+};
+
+struct Grid {
+    int nx, ny, nxt, nyt;
+    double dx, dy, gamma;
+};
+
+// ---------------------------------------------------------------- boundary
+__device__ __forceinline__ void dm_exact(double x, double y, double t, double gamma, double q[4]) {
+    // weno5_hllc_refsym.py:25-54 double_mach_exact_conserved
+    const double root3 = 1.7320508075688772;
+    double x0 = 1.0 / 6.0;
+    double shock_x = x0 + y / root3 + 20.0 * t / root3;
+    double rho, u, v, p;
+    if (x < shock_x) { rho = 8.0; u = 8.25 * root3 * 0.5; v = -8.25 * 0.5; p = 116.5; }
+    else { rho = 1.4; u = 0.0; v = 0.0; p = 1.0; }
+    double e = 0.5 * rho * (u * u + v * v) + p / (gamma - 1.0);
+    q[0] = rho; q[1] = rho * u; q[2] = rho * v; q[3] = e;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_boundary(const double *__restrict__ src, double *__restrict__ dst,
+                                                    Grid g, int boundary, double time_host,
+                                                    const Ctl *ctl, double time_frac) {
+    double time = time_host;
+    if (ctl) {
+        if (ctl->done) return;
+        time = ctl->t + time_frac * ctl->dt;
+    }
+    const int gc = 3;
+    int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= g.nxt * g.nyt) return;
+    int j = idx / g.nxt, i = idx - j * g.nxt;
+    bool inside = i >= gc && i < g.nx + gc && j >= gc && j < g.ny + gc;
+    int si = i, sj = j;
+    double q[4];
+    if (boundary == WSR_BC_PERIODIC) {
+        if (i < gc) si = i + g.nx; else if (i >= g.nx + gc) si = i - g.nx;
+        if (j < gc) sj = j + g.ny; else if (j >= g.ny + gc) sj = j - g.ny;
+    } else {
+        if (i < gc) si = gc; else if (i >= g.nx + gc) si = g.nx + gc - 1;
+        if (j < gc) sj = gc; else if (j >= g.ny + gc) sj = g.ny + gc - 1;
+    }
+    const double2 *s2 = reinterpret_cast<const double2 *>(src + ((size_t)sj * g.nxt + si) * 4);
+    double2 a = s2[0], b = s2[1];
+    q[0] = a.x; q[1] = a.y; q[2] = b.x; q[3] = b.y;
+    if (boundary == WSR_BC_DOUBLE_MACH && !inside) {
+        // weno5_hllc_refsym.py:57-124, applied to the transmissive copy
+        double x = ((double)(i - gc) + 0.5) * g.dx;
+        double y = ((double)(j - gc) + 0.5) * g.dy;
+        const double x0 = 1.0 / 6.0;
+        if (j >= g.ny + gc) {
+            dm_exact(x, y, time, g.gamma, q);
+        } else if (j < gc) {
+            if (x < x0) {
+                dm_exact(x, 0.0, time, g.gamma, q);
+            } else {
+                int rj = 2 * gc - 1 - j;
+                int ri = i < gc ? gc : (i >= g.nx + gc ? g.nx + gc - 1 : i);
+                const double *r = src + ((size_t)rj * g.nxt + ri) * 4;
+                q[0] = r[0]; q[1] = r[1]; q[2] = -r[2]; q[3] = r[3];
+            }
+        } else if (i < gc) {
+            dm_exact(x, y, time, g.gamma, q);
+        }
+        // right boundary: transmissive copy already in q
+    }
+    double2 *d2 = reinterpret_cast<double2 *>(dst + (size_t)idx * 4);
+    d2[0] = make_double2(q[0], q[1]);
+    d2[1] = make_double2(q[2], q[3]);
+}
+
+// ------------------------------------------------------------------ sweep
+struct SweepArgs {
+    Grid g;
+    double dt_host;
+    const Ctl *ctl;
+    int eno;
+};
+
+template <int DIR>
+__device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
+    if (DIR == 1) { fl = it % TN; tau = it / TN; }      // faces fastest (x: contiguous cells)
+    else { tau = it % ntau; fl = it / ntau; }            // columns fastest (y: contiguous cells)
+}
+
+template <int DIR, int SCHEME, int RIEMANN>
+__global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
+                                                 SweepArgs A, typename MlpFor<SCHEME>::type mlp) {
+    double dt = A.dt_host;
+    if (A.ctl) {
+        if (A.ctl->done) return;
+        dt = A.ctl->dt;
+    }
+    const Grid &g = A.g;
+    const double h_n = DIR == 1 ? g.dx : g.dy;   // sweep-axis spacing
+    const double h_t = DIR == 1 ? g.dy : g.dx;   // transverse spacing
+    const double dt_h = dt / h_n;
+    const int nfaces = (DIR == 1 ? g.nx : g.ny) + 1;
+    const int ntrans = DIR == 1 ? g.ny : g.nx;
+    const int f0 = blockIdx.x * TN;
+    const int t0 = blockIdx.y * TT;
+
+    __shared__ double T[NN * NT1 * 4];      // stencil tile [n][tau][c]
+    __shared__ double FV[NT1 * TN * 8];     // face values [tau][face][side*4+k]
+
+    // cooperative tile load, contiguous along the padded row
+    for (int e = threadIdx.x; e < NN * NT1; e += BLOCK) {
+        int n, tau, prow, pcol;
+        if (DIR == 1) { n = e % NN; tau = e / NN; prow = t0 + 1 + tau; pcol = f0 + n; }
+        else { tau = e % NT1; n = e / NT1; prow = f0 + n; pcol = t0 + 1 + tau; }
+        prow = prow < g.nyt ? prow : g.nyt - 1;
+        pcol = pcol < g.nxt ? pcol : g.nxt - 1;
+        const double2 *s2 = reinterpret_cast<const double2 *>(bc + ((size_t)prow * g.nxt + pcol) * 4);
+        double2 a = s2[0], b = s2[1];
+        double *d = T + (n * NT1 + tau) * 4;
+        d[0] = a.x; d[1] = a.y; d[2] = b.x; d[3] = b.y;
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x;
+    const int gi = lane & 7;
+    const int s = gi >> 2;           // 0: state left of the face (right face of cell f-1), 1: right state
+    const int k = gi & 3;            // characteristic / conserved component
+    const int slot = lane >> 3;      // 32 face items per pass
+
+    // ---------------- phase 1: normal characteristic reconstruction
+    constexpr int N1 = NT1 * TN;
+    for (int base = 0; base < N1; base += BLOCK / 8) {
+        int it = base + slot;
+        const bool valid = it < N1;
+        int fl, tau;
+        item_map<DIR>(valid ? it : N1 - 1, NT1, fl, tau);
+        const double *colL = T + ((fl + 2) * NT1 + tau) * 4;
+        const double *colR = T + ((fl + 3) * NT1 + tau) * 4;
+        Eigen E = roe_eigen(colL, colR, g.gamma);
+        double l[4], scale;
+        left_row<DIR>(k, E, g.gamma, l, scale);
+        double c[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const double *q = T + ((fl + s + m) * NT1 + tau) * 4;
+            c[m] = (l[0] * q[0] + l[1] * q[1] + l[2] * q[2] + l[3] * q[3]) * scale;
+        }
+        double val = recon_normal<SCHEME>(c, s == 0, mlp, A.eno, h_n);
+        const int qb = lane & ~3;
+        double cf0 = __shfl(val, qb + 0), cf1 = __shfl(val, qb + 1);
+        double cf2 = __shfl(val, qb + 2), cf3 = __shfl(val, qb + 3);
+        double r[4];
+        right_row<DIR>(k, E, r);
+        double out = r[0] * cf0 + r[1] * cf1 + r[2] * cf2 + r[3] * cf3;
+        if (valid) FV[(tau * TN + fl) * 8 + gi] = out;
+    }
+    __syncthreads();
+
+    // ---------------- phase 2: transverse Gauss reconstruction + Riemann flux
+    constexpr int N2 = TT * TN;
+    for (int base = 0; base < N2; base += BLOCK / 8) {
+        int it = base + slot;
+        const bool valid = it < N2;
+        int fl, tl;
+        item_map<DIR>(valid ? it : N2 - 1, TT, fl, tl);
+        double q[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) q[m] = FV[((tl + m) * TN + fl) * 8 + gi];
+        double g1, g2;
+        recon_gauss<SCHEME>(q, mlp, A.eno, h_t, g1, g2);
+        // lanes with s = 0 evaluate the flux at Gauss point 1, s = 1 at point 2
+        const int gb = lane & ~7;
+        double ul[4], ur[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            double a1 = __shfl(g1, gb + m), a2 = __shfl(g2, gb + m);
+            double b1 = __shfl(g1, gb + 4 + m), b2 = __shfl(g2, gb + 4 + m);
+            ul[m] = s == 0 ? a1 : a2;
+            ur[m] = s == 0 ? b1 : b2;
+        }
+        double f[4];
+        if constexpr (RIEMANN == RIEMANN_FORCE) force_flux<DIR>(ul, ur, dt_h, g.gamma, f);
+        else hllc_flux<DIR>(ul, ur, g.gamma, f);
+        double fk = k == 0 ? f[0] : k == 1 ? f[1] : k == 2 ? f[2] : f[3];
+        double other = __shfl_xor(fk, 4);
+        const int face = f0 + fl, tt = t0 + tl;
+        if (valid && s == 0 && face < nfaces && tt < ntrans) {
+            size_t o = DIR == 1 ? ((size_t)tt * (g.nx + 1) + face) * 4 + k
+                                : ((size_t)face * g.nx + tt) * 4 + k;
+            F[o] = fk * dt_h + other * dt_h;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ update
+__device__ __forceinline__ double cell_speed(const double q[4], double gamma) {
+    // warp_weno5_helpers.py:876-897 compute_max_speed_kernel
+    double rho = dmax(q[0], 1.0e-15);
+    double vx = q[1] / rho, vy = q[2] / rho;
+    double p = (gamma - 1.0) * (q[3] - 0.5 * rho * (vx * vx + vy * vy));
+    p = dmax(p, 1.0e-15);
+    double c = sqrt(gamma * p / rho);
+    return dmax(fabs(vx) + c, fabs(vy) + c);
+}
+
+// block max of non-negative speeds (NaN reported separately) -> atomics in ctl
+__device__ __forceinline__ void reduce_speed(double sp, bool active, Ctl *ctl) {
+    __shared__ unsigned long long red[BLOCK / 64];
+    __shared__ int nan_seen;
+    if (threadIdx.x == 0) nan_seen = 0;
+    __syncthreads();
+    unsigned long long bits = 0ull;
+    if (active) {
+        if (sp != sp) nan_seen = 1;
+        else bits = (unsigned long long)__double_as_longlong(sp);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_down(bits, off);
+        bits = o > bits ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bits;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = red[0];
+        for (int w = 1; w < BLOCK / 64; ++w) m = red[w] > m ? red[w] : m;
+        atomicMax(&ctl->speed_bits, m);
+        if (nan_seen) atomicOr(&ctl->nan_flag, 1);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ us, const double *u0,
+                                                  double *u_out, const double *__restrict__ Fx,
+                                                  const double *__restrict__ Fy, Grid g, int rk,
+                                                  Ctl *ctl, int fold_speed, double *dts) {
+    if (ctl && ctl->done) return;
+    int idx = blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = idx < g.nx * g.ny;
+    double q[4] = {0.0, 0.0, 0.0, 0.0};
+    if (active) {
+        int j = idx / g.nx, i = idx - j * g.nx;
+        size_t c = ((size_t)(j + 3) * g.nxt + (i + 3)) * 4;
+        const double *fxl = Fx + ((size_t)j * (g.nx + 1) + i) * 4;
+        const double *fyb = Fy + ((size_t)j * g.nx + i) * 4;
+        const double *fyt = Fy + ((size_t)(j + 1) * g.nx + i) * 4;
+#pragma unroll
+        for (int comp = 0; comp < 4; ++comp) {
+            double rhs = us[c + comp] - 0.5 * (fxl[4 + comp] - fxl[comp]) - 0.5 * (fyt[comp] - fyb[comp]);
+            double o;
+            if (rk == 1) o = rhs;
+            else if (rk == 2) o = 0.75 * u0[c + comp] + 0.25 * rhs;
+            else o = (1.0 / 3.0) * u0[c + comp] + (2.0 / 3.0) * rhs;
+            q[comp] = o;
+        }
+        for (int comp = 0; comp < 4; ++comp) u_out[c + comp] = q[comp];
+    }
+    if (fold_speed) {
+        reduce_speed(active ? cell_speed(q, g.gamma) : 0.0, active, ctl);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            // run_to_time bookkeeping: t += dt; step += 1; dts.append(dt)
+            double dt = ctl->dt;
+            dts[ctl->step] = dt;
+            ctl->t = ctl->t + dt;
+            ctl->dt_last = dt;
+            ctl->step = ctl->step + 1;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_max_speed(const double *__restrict__ u, Grid g, Ctl *ctl) {
+    int idx = blockIdx.x * BLOCK + threadIdx.x;
+    const bool active = idx < g.nx * g.ny;
+    double sp = 0.0;
+    if (active) {
+        int j = idx / g.nx, i = idx - j * g.nx;
+        const double *c = u + ((size_t)(j + 3) * g.nxt + (i + 3)) * 4;
+        double q[4] = {c[0], c[1], c[2], c[3]};
+        sp = cell_speed(q, g.gamma);
+    }
+    reduce_speed(sp, active, ctl);
+}
+
+__global__ void k_reset_speed(Ctl *ctl) {
+    ctl->speed_bits = 0ull;
+    ctl->nan_flag = 0;
+}
+
+__global__ void k_loop_init(Ctl *ctl, double t_end, double cfl_h, int max_steps) {
+    ctl->t = 0.0; ctl->dt = 0.0; ctl->t_end = t_end; ctl->cfl_h = cfl_h; ctl->dt_last = 0.0;
+    ctl->speed_bits = 0ull; ctl->step = 0; ctl->done = 0; ctl->nan_flag = 0; ctl->max_steps = max_steps;
+}
+
+// dt of the next step (warp_weno5_helpers.py:189-207 and the loop head of run_to_time)
+__global__ void k_dt(Ctl *ctl) {
+    if (ctl->done) return;
+    double t = ctl->t;
+    if (!(t < ctl->t_end - 1.0e-14) || ctl->step >= ctl->max_steps) { ctl->done = 1; return; }
+    double dt;
+    if (ctl->nan_flag) {
+        dt = __longlong_as_double(0x7ff8000000000000ll);
+    } else {
+        double max_speed = __longlong_as_double((long long)ctl->speed_bits);
+        dt = max_speed < 1.0e-15 ? 1.0e10 : ctl->cfl_h / max_speed;
+    }
+    double rem = ctl->t_end - t;
+    dt = rem < dt ? rem : dt;   // python min(dt, rem)
+    if (!isfinite(dt) || dt <= 0.0) { ctl->done = 1; return; }
+    ctl->dt = dt;
+    ctl->speed_bits = 0ull;
+    ctl->nan_flag = 0;
+}
+
+// ----------------------------------------------------------- host side
+struct Workspace {
+    Ctl *ctl;
+    double *bc, *Fx, *Fy;
+};
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+static size_t ws_layout(int nx, int ny, Workspace *w, void *base) {
+    size_t nstate = (size_t)(nx + 6) * (ny + 6) * 4 * sizeof(double);
+    size_t nfx = (size_t)ny * (nx + 1) * 4 * sizeof(double);
+    size_t nfy = (size_t)(ny + 1) * nx * 4 * sizeof(double);
+    size_t off = 0;
+    char *b = (char *)base;
+    if (w) w->ctl = (Ctl *)(b + off);
+    off += align256(sizeof(Ctl));
+    if (w) w->bc = (double *)(b + off);
+    off += align256(nstate);
+    if (w) w->Fx = (double *)(b + off);
+    off += align256(nfx);
+    if (w) w->Fy = (double *)(b + off);
+    off += align256(nfy);
+    return off;
+}
+
+static float host_round_bf16(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return x;
+    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+    float y;
+    memcpy(&y, &u, 4);
+    return y;
+}
+
+static int check_cfg(const wsr_euler2d_config *cfg) {
+    if (!cfg) return WSR_E_ARG;
+    if (cfg->ghost != 3 || cfg->nx < 1 || cfg->ny < 1) return WSR_E_ARG;
+    if (cfg->nx > (1 << 15) || cfg->ny > (1 << 15)) return WSR_E_ARG;
+    if (cfg->scheme < WSR_SCHEME_JS || cfg->scheme > WSR_SCHEME_Z) return WSR_E_ARG;
+    if (cfg->riemann != WSR_RIEMANN_HLLC && cfg->riemann != WSR_RIEMANN_FORCE) return WSR_E_ARG;
+    if (cfg->boundary < WSR_BC_TRANSMISSIVE || cfg->boundary > WSR_BC_DOUBLE_MACH) return WSR_E_ARG;
+    bool sr = cfg->scheme == WSR_SCHEME_SR64 || cfg->scheme == WSR_SCHEME_SR32 || cfg->scheme == WSR_SCHEME_SRBF16;
+    if (sr && !cfg->mlp) return WSR_E_MLP;
+    return WSR_OK;
+}
+
+static Grid make_grid(const wsr_euler2d_config *cfg) {
+    Grid g;
+    g.nx = cfg->nx; g.ny = cfg->ny; g.nxt = cfg->nx + 6; g.nyt = cfg->ny + 6;
+    g.dx = cfg->dx; g.dy = cfg->dy; g.gamma = cfg->gamma;
+    return g;
+}
+
+template <int DIR, int SCHEME, int RIEMANN>
+static void launch_sweep_t(const double *bc, double *F, const SweepArgs &A, const void *mlp_host,
+                           hipStream_t st) {
+    using MLP = typename MlpFor<SCHEME>::type;
+    MLP mlp;
+    memset(&mlp, 0, sizeof mlp);
+    if constexpr (SCHEME == SCHEME_SR64) {
+        memcpy(mlp.p, mlp_host, sizeof mlp.p);
+    } else if constexpr (SCHEME == SCHEME_SR32) {
+        memcpy(mlp.p, mlp_host, sizeof mlp.p);
+    } else if constexpr (SCHEME == SCHEME_SRBF16) {
+        const float *src = (const float *)mlp_host;
+        for (int i = 0; i < MLP5_COUNT; ++i) mlp.p[i] = host_round_bf16(src[i]);
+    }
+    const int nfaces = (DIR == 1 ? A.g.nx : A.g.ny) + 1;
+    const int ntrans = DIR == 1 ? A.g.ny : A.g.nx;
+    dim3 grid((nfaces + TN - 1) / TN, (ntrans + TT - 1) / TT);
+    hipLaunchKernelGGL((k_sweep<DIR, SCHEME, RIEMANN>), grid, dim3(BLOCK), 0, st, bc, F, A, mlp);
+}
+
+template <int DIR>
+static void launch_sweep(int scheme, int riemann, const double *bc, double *F, const SweepArgs &A,
+                         const void *mlp_host, hipStream_t st) {
+#define WSR_CASE(S)                                                                              \
+    case S:                                                                                      \
+        if (riemann == RIEMANN_FORCE) launch_sweep_t<DIR, S, RIEMANN_FORCE>(bc, F, A, mlp_host, st); \
+        else launch_sweep_t<DIR, S, RIEMANN_HLLC>(bc, F, A, mlp_host, st);                       \
+        break;
+    switch (scheme) {
+        WSR_CASE(SCHEME_JS)
+        WSR_CASE(SCHEME_SR64)
+        WSR_CASE(SCHEME_SR32)
+        WSR_CASE(SCHEME_SRBF16)
+        WSR_CASE(SCHEME_Z)
+    }
+#undef WSR_CASE
+}
+
+// one stage; ctl != nullptr selects the device-resident dt / time
+static void enqueue_stage(const wsr_euler2d_config *cfg, const Workspace &w, const double *u_stage,
+                          const double *u0, double *u_out, double dt, int rk, double boundary_time,
+                          Ctl *ctl, int fold_speed, double *dts, hipStream_t st) {
+    Grid g = make_grid(cfg);
+    const int ncells_p = g.nxt * g.nyt;
+    double frac = rk == 1 ? 0.0 : (rk == 2 ? 1.0 : 0.5);
+    hipLaunchKernelGGL(k_boundary, dim3((ncells_p + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, u_stage, w.bc, g,
+                       cfg->boundary, boundary_time, (const Ctl *)ctl, frac);
+    SweepArgs A;
+    A.g = g; A.dt_host = dt; A.ctl = ctl; A.eno = cfg->eno_cutoff;
+    launch_sweep<1>(cfg->scheme, cfg->riemann, w.bc, w.Fx, A, cfg->mlp, st);
+    launch_sweep<2>(cfg->scheme, cfg->riemann, w.bc, w.Fy, A, cfg->mlp, st);
+    const int ncells = g.nx * g.ny;
+    hipLaunchKernelGGL(k_update, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const double *)w.bc, u0,
+                       u_out, (const double *)w.Fx, (const double *)w.Fy, g, rk, ctl, fold_speed, dts);
+}
+
+}  // namespace wsr
+
+using namespace wsr;
+
+extern "C" {
+
+size_t wsr_euler2d_workspace_bytes(int32_t nx, int32_t ny, int32_t ghost) {
+    if (ghost != 3 || nx < 1 || ny < 1) return 0;
+    return ws_layout(nx, ny, nullptr, nullptr);
+}
+
+int wsr_euler2d_stage(const wsr_euler2d_config *cfg, const double *u_stage, const double *u0, double *u_out,
+                      double dt, int32_t rk, double boundary_time, void *workspace, size_t workspace_bytes,
+                      void *stream) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (!u_stage || !u0 || !u_out || !workspace || rk < 1 || rk > 3) return WSR_E_ARG;
+    if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
+    Workspace w;
+    ws_layout(cfg->nx, cfg->ny, &w, workspace);
+    hipStream_t st = (hipStream_t)stream;
+    enqueue_stage(cfg, w, u_stage, u0, u_out, dt, rk, boundary_time, nullptr, 0, nullptr, st);
+    return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler2d_max_speed(const wsr_euler2d_config *cfg, const double *u, void *workspace, size_t workspace_bytes,
+                          double *max_speed_out, void *stream) {
+    if (!cfg || cfg->ghost != 3 || cfg->nx < 1 || cfg->ny < 1) return WSR_E_ARG;
+    if (!u || !workspace || !max_speed_out) return WSR_E_ARG;
+    if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
+    Workspace w;
+    ws_layout(cfg->nx, cfg->ny, &w, workspace);
+    hipStream_t st = (hipStream_t)stream;
+    Grid g = make_grid(cfg);
+    hipLaunchKernelGGL(k_reset_speed, dim3(1), dim3(1), 0, st, w.ctl);
+    const int ncells = g.nx * g.ny;
+    hipLaunchKernelGGL(k_max_speed, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, u, g, w.ctl);
+    Ctl host;
+    if (hipMemcpyAsync(&host, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st) != hipSuccess) return WSR_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return WSR_E_HIP;
+    double m;
+    memcpy(&m, &host.speed_bits, sizeof m);
+    *max_speed_out = host.nan_flag ? NAN : m;
+    return WSR_OK;
+}
+
+int wsr_euler2d_loop_init(const wsr_euler2d_config *cfg, const double *u, int32_t max_steps, void *workspace,
+                          size_t workspace_bytes, void *stream) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (!u || !workspace || max_steps < 0) return WSR_E_ARG;
+    if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
+    Workspace w;
+    ws_layout(cfg->nx, cfg->ny, &w, workspace);
+    hipStream_t st = (hipStream_t)stream;
+    Grid g = make_grid(cfg);
+    double h = cfg->dx < cfg->dy ? cfg->dx : cfg->dy;
+    hipLaunchKernelGGL(k_loop_init, dim3(1), dim3(1), 0, st, w.ctl, cfg->t_end, cfg->cfl * h, max_steps);
+    const int ncells = g.nx * g.ny;
+    hipLaunchKernelGGL(k_max_speed, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, u, g, w.ctl);
+    return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1, double *u2, double *dts,
+                           int32_t n_steps, void *workspace, size_t workspace_bytes, void *stream) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (!u || !u1 || !u2 || !dts || !workspace || n_steps < 0) return WSR_E_ARG;
+    if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
+    Workspace w;
+    ws_layout(cfg->nx, cfg->ny, &w, workspace);
+    hipStream_t st = (hipStream_t)stream;
+    for (int n = 0; n < n_steps; ++n) {
+        hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
+        enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st);
+        enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st);
+        enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st);
+    }
+    return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler2d_loop_status(void *workspace, int32_t *steps, double *t, double *dt_last, int32_t *done,
+                            void *stream) {
+    if (!workspace) return WSR_E_ARG;
+    Workspace w;
+    w.ctl = (Ctl *)workspace;
+    hipStream_t st = (hipStream_t)stream;
+    Ctl host;
+    if (hipMemcpyAsync(&host, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st) != hipSuccess) return WSR_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return WSR_E_HIP;
+    if (steps) *steps = host.step;
+    if (t) *t = host.t;
+    if (dt_last) *dt_last = host.dt_last;
+    if (done) *done = host.done;
+    return WSR_OK;
+}
+
+const char *wsr_version(void) { return "wsr-hip 0.1 gfx950"; }
+
+}  // extern "C"

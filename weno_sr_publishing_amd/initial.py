@@ -1,0 +1,131 @@
+"""Initial-condition hooks (host numpy, excluded from timing like the reference).
+
+* Four-quadrant 2-D Riemann problems C.3-C.6:
+  code/snapshot/for_paper_results/run_quadrant.py:20-106 (states, region rule,
+  tensor-product Gauss-Legendre cell averages via numpy's ``leggauss``).
+* Planar Sod strip at t=0: code/snapshot/teacherfree_lab_weno5/warp_sod_validation.py:48-62,176-210
+  (15-point cell averages with the reference's own quadrature table).
+* ``pad_periodic``: code/snapshot/for_paper_results/run_vortex.py:34-35.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .params import GAUSS15_W, GAUSS15_XI
+
+# state order: 1 = upper right, 2 = upper left, 3 = lower left, 4 = lower right
+CASES = {
+    "c3": {"label": "C.3 (q400)", "t_end": 0.50,
+           "states": ((1.5, 0.0, 0.0, 1.5), (0.5323, 1.206, 0.0, 0.3),
+                      (0.138, 1.206, 1.206, 0.029), (0.5323, 0.0, 1.206, 0.3))},
+    "c4": {"label": "C.4", "t_end": 0.25,
+           "states": ((1.1, 0.0, 0.0, 1.1), (0.5065, 0.8939, 0.0, 0.35),
+                      (1.1, 0.8939, 0.8939, 1.1), (0.5065, 0.0, 0.8939, 0.35))},
+    "c5": {"label": "C.5", "t_end": 0.23,
+           "states": ((1.0, -0.75, -0.5, 1.0), (2.0, -0.75, 0.5, 1.0),
+                      (1.0, 0.75, 0.5, 1.0), (3.0, 0.75, -0.5, 1.0))},
+    "c6": {"label": "C.6", "t_end": 0.30,
+           "states": ((1.0, 0.75, -0.5, 1.0), (2.0, 0.75, 0.5, 1.0),
+                      (1.0, -0.75, 0.5, 1.0), (3.0, -0.75, -0.5, 1.0))},
+}
+
+
+def conserved_from_primitive(rho, u, v, p, gamma):
+    out = np.empty(np.shape(rho) + (4,), dtype=np.float64)
+    out[..., 0] = rho
+    out[..., 1] = rho * u
+    out[..., 2] = rho * v
+    out[..., 3] = p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)
+    return out
+
+
+def quadrant_primitive(x, y, states):
+    if len(states) != 4:
+        raise ValueError("a quadrant problem requires four primitive states")
+    rho = np.empty_like(x, dtype=np.float64)
+    vx = np.empty_like(x, dtype=np.float64)
+    vy = np.empty_like(x, dtype=np.float64)
+    pressure = np.empty_like(x, dtype=np.float64)
+    regions = ((x >= 0.5) & (y >= 0.5), (x < 0.5) & (y >= 0.5),
+               (x < 0.5) & (y < 0.5), (x >= 0.5) & (y < 0.5))
+    for mask, (rho_q, vx_q, vy_q, p_q) in zip(regions, states):
+        rho[mask] = rho_q
+        vx[mask] = vx_q
+        vy[mask] = vy_q
+        pressure[mask] = p_q
+    return rho, vx, vy, pressure
+
+
+def make_quadrant_state(nx: int, ny: int, ghost: int, definition: dict, quadrature: int = 15,
+                        x_min: float = 0.0, x_max: float = 1.0, y_min: float = 0.0,
+                        y_max: float = 1.0, gamma: float = 1.4) -> np.ndarray:
+    """Tensor-product Gauss-Legendre cell averages on the padded grid."""
+    if quadrature < 1:
+        raise ValueError("quadrature order must be at least one")
+    dx = (x_max - x_min) / nx
+    dy = (y_max - y_min) / ny
+    xi, wi = np.polynomial.legendre.leggauss(quadrature)
+    jj, ii = np.indices((ny + 2 * ghost, nx + 2 * ghost))
+    xc = x_min + (ii - ghost + 0.5) * dx
+    yc = y_min + (jj - ghost + 0.5) * dy
+    state = np.zeros((ny + 2 * ghost, nx + 2 * ghost, 4), dtype=np.float64)
+    for sx, wx in zip(xi, wi):
+        x = xc + 0.5 * dx * float(sx)
+        for sy, wy in zip(xi, wi):
+            y = yc + 0.5 * dy * float(sy)
+            rho, vx, vy, pressure = quadrant_primitive(x, y, definition["states"])
+            state += 0.25 * float(wx) * float(wy) * conserved_from_primitive(rho, vx, vy, pressure, gamma)
+    return state
+
+
+def pad_periodic(interior_state: np.ndarray, ghost: int) -> np.ndarray:
+    return np.pad(interior_state, ((ghost, ghost), (ghost, ghost), (0, 0)), mode="wrap")
+
+
+def make_quadrant_initial(nx: int, ny: int, case: str = "c4", ghost: int = 3,
+                          quadrature: int = 15) -> np.ndarray:
+    """Initial state exactly as the timing harness builds it
+    (code/snapshot/for_paper_results/run_weno5_timing.py:86-104): averages on the
+    unit square, interior re-padded periodically to ``ghost`` cells."""
+    definition = CASES[case]
+    full = make_quadrant_state(nx, ny, 4, definition, quadrature)
+    inner = full[4: 4 + ny, 4: 4 + nx, :]
+    return pad_periodic(inner, ghost)
+
+
+def _sod_states():
+    return (1.0, 0.0, 1.0), (0.125, 0.0, 0.1)
+
+
+def _primitive_to_conserved_axis(rho, normal_vel, pressure, gamma, axis):
+    energy = 0.5 * rho * normal_vel * normal_vel + pressure / (gamma - 1.0)
+    if axis == "y":
+        return np.array([rho, 0.0, rho * normal_vel, energy], dtype=np.float64)
+    return np.array([rho, rho * normal_vel, 0.0, energy], dtype=np.float64)
+
+
+def _cell_average_sod0(coord_center: float, h: float, gamma: float, axis: str) -> np.ndarray:
+    left, right = _sod_states()
+    state = np.zeros(4, dtype=np.float64)
+    for xi, weight in zip(GAUSS15_XI, GAUSS15_W):
+        coord = coord_center + 0.5 * h * float(xi)
+        rho, vel, pressure = left if coord < 0.0 else right
+        state += float(weight) * _primitive_to_conserved_axis(rho, vel, pressure, gamma, axis)
+    return 0.5 * state
+
+
+def make_sod_state(params, axis: str = "x") -> np.ndarray:
+    """Sod strip centred on 0 at t = 0 (warp_sod_validation.py:191-210)."""
+    ny_total, nx_total, _ = params.padded_shape
+    u = np.zeros(params.padded_shape, dtype=np.float64)
+    if axis == "y":
+        y0 = -0.5 * params.y_length
+        con = np.array([_cell_average_sod0(y0 + (j - params.ghost + 0.5) * params.dy, params.dy,
+                                           params.gamma, axis) for j in range(ny_total)])
+        u[:, :, :] = con[:, None, :]
+    else:
+        x0 = -0.5 * params.x_length
+        con = np.array([_cell_average_sod0(x0 + (i - params.ghost + 0.5) * params.dx, params.dx,
+                                           params.gamma, axis) for i in range(nx_total)])
+        u[:, :, :] = con[None, :, :]
+    return u
