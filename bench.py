@@ -1,0 +1,263 @@
+#!/usr/bin/env python3
+"""Benchmark: 2-D Euler cell-updates/s of WENO5-SR vs WENO5-JS at 400^2 (one MI355X).
+
+Workload: the four-quadrant Riemann problem C.4 at 400x400 (15x15 Gauss-Legendre
+cell averages, transmissive boundaries), characteristic WENO5 + HLLC + SSPRK3,
+CFL 0.4, FP64 state; closures WENO5-SR with the FP32 MLP (headline), WENO5-JS
+(classical baseline), WENO5-SR FP64 / BF16.  One "step" is one SSPRK3 step over
+the 400x400 grid (3 stages); one cell-update is one interior cell advanced by one
+step.  `--steps 0` (default) times the whole trajectory to t = 0.25, the protocol
+of the reference's timing harness (for_paper_results/run_weno5_timing.py:111-140).
+
+Prints ONE JSON line (rank 0).  Extra keys: `js` (WENO5-JS throughput on the same
+trajectory), `sr_vs_js_wall`, `accuracy` (L1 density error of each 400^2 run
+against an 800^2 FP64 WENO5-JS solution block-averaged to 400^2), `roofline`
+(sweep kernels, HIP-event timed inside the timed region) and `cpu_baseline`
+(the CPU oracle on the host cores, bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "2D Euler cell-updates/s, WENO5-SR vs WENO-JS at 400^2, same L1 error"
+T_END = 0.25
+NX = NY = 400
+PEAK_VALU_TFLOPS = 78.6     # MI355X FP64 vector / non-packed FP32 vector rate
+PEAK_HBM_GBS = 8000.0
+REFERENCE_SR32_WALL_S = 14.253   # experiments/14_mixed_precision_timing/tables/weno5_precision_timing.tex:11
+REFERENCE_JS_WALL_S = 6.605      # same table, line 9
+
+# ---------------------------------------------------------------- flop model
+# Algorithmic operations per unit (add/sub/mul/div/sqrt/exp/log/tanh = 1 each;
+# comparisons, min/max/abs and sign flips = 0), counted from the reference
+# expressions (warp_weno5_helpers.py, weno5_rk3_diff_mlp_f32.py, weno5_hllc.py):
+MLP_RAW = 415            # 5-10-6-6-3 MLP: 346 affine + 66 swish + 9 tanh head + 12 softmax
+SR_CLOSURE = 2 * MLP_RAW + 25 + 6 + 12   # + non-plateau features, reflection average, 1/(3r+eps)^2
+FEAT_ALWAYS = 28         # second differences, deltas, plateau test
+NORMAL = {"js": 70, "z": 75,
+          "sr": (FEAT_ALWAYS + 15 + 5, FEAT_ALWAYS + 15 + 14 + SR_CLOSURE)}   # (plateau, closure)
+GAUSS = {"js": 101, "z": 106,
+         "sr": (FEAT_ALWAYS + 30 + 10, FEAT_ALWAYS + 30 + 28 + SR_CLOSURE)}
+FACE_LINE_FIXED = 68 + 42 + 40 * 8 + 8 * 7   # Roe + eigenvalues, left rows, 40 projections, back-projection
+FLUX_FACE_FIXED = 2 * 110 + 12               # two HLLC fluxes + dt/h scaling
+BYTES_PER_SWEEP_CELL = 32 + 32               # state tile read + flux write (FP64 x 4)
+
+
+def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -> float:
+    nfaces, ntrans = NX + 1, NY
+    if scheme.startswith("sr"):
+        n = NORMAL["sr"][0] + frac_normal * (NORMAL["sr"][1] - NORMAL["sr"][0])
+        g = GAUSS["sr"][0] + frac_gauss * (GAUSS["sr"][1] - GAUSS["sr"][0])
+    else:
+        n, g = NORMAL[scheme], GAUSS[scheme]
+    return nfaces * (ntrans + 4) * (FACE_LINE_FIXED + 8 * n) + nfaces * ntrans * (FLUX_FACE_FIXED + 8 * g)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0, help="timed SSPRK3 steps (0 = full trajectory to t=0.25)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--closure", default="sr32", choices=["sr32", "sr64", "srbf16"])
+    ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
+    return ap.parse_args()
+
+
+def main() -> int:
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU only: the WENO5 step is measured on "
+                                   "one MI355X (multi-GPU domain decomposition is design-only)",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+
+    import torch
+    from weno_sr_publishing_amd.checkpoints import load_f32, load_f64
+    from weno_sr_publishing_amd.common import conservative_block_average, rho_errors, state_health
+    from weno_sr_publishing_amd.euler2d import Euler2DSolver
+    from weno_sr_publishing_amd.initial import make_quadrant_initial
+    from weno_sr_publishing_amd.params import Params
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}), flush=True)
+        return 1
+    dev = torch.device("cuda:0")
+    params = Params(nx=NX, ny=NY, x_length=1.0, y_length=1.0, cfl=0.4, t_end=T_END)
+    u0_host = make_quadrant_initial(NX, NY, "c4")
+    mlp32 = load_f32()
+    mlps = {"sr32": mlp32, "sr64": load_f64(), "srbf16": mlp32.as_precision("bf16"), "js": None}
+
+    def make(scheme, p=params):
+        return Euler2DSolver(p, scheme, "hllc", "transmissive", mlps.get(scheme), False, dev)
+
+    def timed_run(scheme: str, max_steps: int, instrument: bool):
+        """Warm-up, then the timed region: t = 0 .. t_end (or max_steps) from the resident IC."""
+        s = make(scheme)
+        u_init = s.new_state(u0_host)
+        u = torch.empty_like(u_init)
+        if args.warmup > 0:
+            u.copy_(u_init)
+            s.run(u, T_END, max_steps=args.warmup)
+        u.copy_(u_init)
+        torch.cuda.synchronize()
+        sweep_ms, sweep_n = 0.0, 0
+        t0 = time.perf_counter()
+        s.loop_init(u, T_END, max_steps)
+        while True:
+            if instrument:
+                ms, n = s.loop_steps_timed(u, 64)
+                sweep_ms += ms
+                sweep_n += n
+            else:
+                s.loop_steps(u, 64)
+            steps, t, _, done = s.loop_status()
+            if done:
+                break
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return s, u, steps, t, wall, sweep_ms, sweep_n
+
+    max_steps = args.steps if args.steps > 0 else 10_000_000
+    s_sr, u_sr, steps_sr, t_sr, wall_sr, sweep_ms, sweep_n = timed_run(args.closure, max_steps, True)
+    _, u_js, steps_js, t_js, wall_js, js_sweep_ms, js_sweep_n = timed_run("js", max_steps, True)
+    health = state_health(u_sr.cpu().numpy(), 3, NX, NY)
+    if not health["complete"]:
+        print(json.dumps({"error": "WENO5-SR run did not complete", "health": health}), flush=True)
+        return 1
+
+    value = NX * NY * steps_sr / wall_sr
+    js_value = NX * NY * steps_js / wall_js
+
+    # closure-evaluation counts along the same (deterministic) trajectory, untimed
+    s_cnt = make(args.closure)
+    s_cnt.set_counting(True)
+    u_cnt = s_cnt.new_state(u0_host)
+    s_cnt.run(u_cnt, T_END, max_steps=max_steps)
+    c = s_cnt.counters()
+    frac_n = c[1] / max(1, c[0])
+    frac_g = c[3] / max(1, c[2])
+    flops = sweep_flops_per_launch(args.closure, frac_n, frac_g)
+    avg_ms = sweep_ms / max(1, sweep_n)
+    achieved = flops / (avg_ms * 1e-3) / 1e12
+    js_avg_ms = js_sweep_ms / max(1, js_sweep_n)
+    js_achieved = sweep_flops_per_launch("js", 1.0, 1.0) / (js_avg_ms * 1e-3) / 1e12
+    hbm_gbs = (NX + 1) * (NY + 4) * BYTES_PER_SWEEP_CELL / (avg_ms * 1e-3) / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps_sr,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_sr / steps_sr,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: C.4 four-quadrant Riemann initial data (15x15 Gauss-Legendre cell averages); "
+                "trained WENO5-SR checkpoints",
+        "config": {"workload": f"2D Euler C.4 400x400, WENO5-SR ({args.closure} closure) + HLLC + SSPRK3, "
+                               f"CFL 0.4, t=0..{T_END} ({'full trajectory' if args.steps <= 0 else f'{steps_sr} steps'})",
+                   "nx": NX, "ny": NY, "closure": args.closure, "state": "fp64", "riemann": "hllc",
+                   "boundary": "transmissive", "t_end": T_END if args.steps <= 0 else t_sr},
+        "wall_s": wall_sr,
+        "js": {"value": js_value, "steps": steps_js, "wall_s": wall_js, "ms_per_step": 1e3 * wall_js / steps_js},
+        "sr_vs_js_wall": wall_sr / wall_js,
+        "reference_rtx5080_wall_s": {"sr32": REFERENCE_SR32_WALL_S, "js": REFERENCE_JS_WALL_S},
+        "speedup_vs_reference_wall": ({"sr": REFERENCE_SR32_WALL_S / wall_sr, "js": REFERENCE_JS_WALL_S / wall_js}
+                                      if args.steps <= 0 and args.closure == "sr32" else None),
+        "roofline": {
+            "bound": "mfma",
+            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
+                       "FP64-vector / non-packed FP32-vector rate",
+            "kernel": f"k_sweep<x|y,{args.closure},hllc>",
+            "achieved": achieved,
+            "peak": PEAK_VALU_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_VALU_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": avg_ms,
+            "launches": sweep_n,
+            "algorithmic_flops_per_launch": flops,
+            "closure_fraction": {"normal": frac_n, "gauss": frac_g},
+            "hbm_algorithmic_GBs": hbm_gbs,
+            "hbm_frac": hbm_gbs / PEAK_HBM_GBS,
+            "js_sweep": {"avg_launch_ms": js_avg_ms, "achieved": js_achieved, "frac": js_achieved / PEAK_VALU_TFLOPS},
+        },
+    }
+
+    if not args.no_accuracy:
+        p8 = Params(nx=2 * NX, ny=2 * NY, x_length=1.0, y_length=1.0, cfl=0.4, t_end=T_END)
+        s8 = make("js", p8)
+        u8 = s8.new_state(make_quadrant_initial(2 * NX, 2 * NY, "c4"))
+        s8.run(u8, T_END)
+        ref = conservative_block_average(u8.cpu().numpy()[3:-3, 3:-3], NX)[..., 0]
+        acc = {}
+        for scheme in ("js", "sr32", "sr64", "srbf16"):
+            if scheme == args.closure and args.steps <= 0:
+                field = u_sr
+            elif scheme == "js" and args.steps <= 0:
+                field = u_js
+            else:
+                s = make(scheme)
+                field = s.new_state(u0_host)
+                s.run(field, T_END)
+            acc[scheme] = rho_errors(field.cpu().numpy()[3:-3, 3:-3, 0], ref)["rho_l1"]
+        out["accuracy"] = {"rho_l1_vs_800sq_fp64_js": acc,
+                           "sr_equal_or_lower_l1_than_js": acc[args.closure] <= acc["js"]}
+
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        # sample: steps of the developed flow (the GPU state half-way through the trajectory)
+        s_mid = make(args.closure)
+        u_mid = s_mid.new_state(u0_host)
+        s_mid.run(u_mid, T_END, max_steps=max(1, steps_sr // 2))
+        mid = u_mid.cpu().numpy()
+        mlp = mlps[args.closure]
+        done_steps, t0 = 0, time.perf_counter()
+        u = mid
+        while True:
+            u, _, n, _ = oracle.run2d(u, NX, NY, params.dx, params.dy, params.gamma, params.cfl, T_END,
+                                      scheme=args.closure, mlp=mlp, max_steps=1)
+            done_steps += n
+            el = time.perf_counter() - t0
+            if n == 0 or el >= args.cpu_seconds or done_steps >= 200:
+                break
+        cpu_rate = NX * NY * done_steps / el
+        cpu_model = platform.processor() or ""
+        try:
+            for line in Path("/proc/cpuinfo").read_text().splitlines():
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{done_steps} SSPRK3 steps of C.4 400x400 ({args.closure} closure) from "
+                                         f"the t~0.125 state, OpenMP C oracle, {el:.1f} s",
+                               "cpu_model": cpu_model, "host_cpus": os.cpu_count()}
+        out["gpu_vs_cpu"] = value / cpu_rate
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -49,13 +49,18 @@ enum {
     WSR_E_HIP = -4         /* a HIP runtime call failed */
 };
 
+/* instrumentation flags (wsr_euler2d_config.flags) */
+enum {
+    WSR_FLAG_COUNT_CLOSURE = 1  /* count reconstructions and closure (MLP) evaluations into the workspace */
+};
+
 /* Grid and method description (a plain C struct; mirrors the fields of
  * Params, code/snapshot/warp_weno5_helpers.py:66-87, plus the method switches). */
 typedef struct wsr_euler2d_config {
     int32_t nx, ny, ghost;          /* ghost must be 3 */
     int32_t scheme, riemann, boundary;
     int32_t eno_cutoff;             /* 0/1, SR only (weno5_rk3_diff_v20_deploy.py:258-279) */
-    int32_t reserved;
+    int32_t flags;                  /* WSR_FLAG_* */
     double dx, dy, gamma;
     double cfl, t_end;              /* used by the device-resident time loop only */
     /* host pointer to the 189 closure parameters w1,b1,w2,b2,w3,b3,w4,b4
@@ -104,6 +109,59 @@ int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1,
                            void *stream);
 int wsr_euler2d_loop_status(void *workspace, int32_t *steps, double *t, double *dt_last,
                             int32_t *done, void *stream);
+
+/* Instrumented loop_steps (no reference counterpart; used by the benchmark):
+ * the same n_steps (<= 64) steps with a HIP event pair around every sweep-kernel
+ * launch on `stream`; synchronises and returns the summed sweep-kernel time of
+ * the launches that ran and their number. */
+int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, double *u1, double *u2,
+                                 double *dts, int32_t n_steps, void *workspace, size_t workspace_bytes,
+                                 void *stream, double *sweep_ms, int32_t *sweep_launches);
+
+/* Closure counters accumulated since loop_init when WSR_FLAG_COUNT_CLOSURE is set:
+ * out4 = {normal reconstructions, of which closure evaluated, Gauss-pair
+ * reconstructions, of which closure evaluated} (synchronises the stream). */
+int wsr_euler2d_counters(void *workspace, uint64_t *out4, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* 1-D periodic scalar advection (the training / GSTE path)                  */
+/* ------------------------------------------------------------------------ */
+typedef struct wsr_adv1d_config {
+    int32_t batch, n;               /* trajectories x cells (5 <= n <= 2048) */
+    int32_t scheme;                 /* WSR_SCHEME_JS / SR64 / SR32 / SRBF16 */
+    int32_t reserved;
+    double dxinv;                   /* 1 / dx */
+    const void *mlp;                /* host pointer to the 189 parameters (as in wsr_euler2d_config) */
+} wsr_adv1d_config;
+
+/* Advance u (device, batch x n, FP64, in place) by n_steps signed SSPRK3 steps of
+ * q_t + a q_x = 0 with velocities[b] in {-1, +1} (device) and the step sizes
+ * dts[n_steps] (device).  Replaces ssprk3_step_signed / rollout_distance of
+ * teacherfree_lab_weno5_v20_distance_balanced_mlp_f32_fast/rk3_advection.py:82-186
+ * and ssprk3 of teacherfree_lab_weno5/apost_advect.py:41-45 (GSTE).  If
+ * checkpoints is not NULL it receives the state at the start of every step and
+ * the final state: (batch, n_steps + 1, n) doubles. */
+int wsr_adv1d_rollout(const wsr_adv1d_config *cfg, double *u, const double *velocities,
+                      const double *dts, int32_t n_steps, double *checkpoints, void *stream);
+
+/* Training (no counterpart in the reference's C/CUDA: the reference differentiates
+ * with torch autograd through checkpointed rollouts,
+ * teacherfree_lab_weno5_v20_distance_balanced_mlp_f32_fast/fast_losses.py:412-648).
+ * Parameters are read from device memory (params_device: 189 doubles for SR64,
+ * floats for SR32) so an optimiser can update them in place.
+ *   train_forward: rollout as wsr_adv1d_rollout, checkpoints required
+ *                  ((batch, n_steps + 1, n) doubles).
+ *   train_adjoint: reverse pass of the same rollout from the adjoint lam_final
+ *                  (batch x n) of the final state: grad (batch x 189 doubles,
+ *                  per-trajectory parameter gradients) and, if lam_initial is not
+ *                  NULL, the adjoint of the initial state.  n <= 1024 (SR32) / 400 (SR64). */
+int wsr_adv1d_train_forward(const wsr_adv1d_config *cfg, double *u, const double *velocities,
+                            const double *dts, int32_t n_steps, double *checkpoints,
+                            const void *params_device, void *stream);
+int wsr_adv1d_train_adjoint(const wsr_adv1d_config *cfg, const double *checkpoints,
+                            const double *lam_final, const double *velocities, const double *dts,
+                            int32_t n_steps, const void *params_device, double *grad,
+                            double *lam_initial, void *stream);
 
 /* Library version / build identification. */
 const char *wsr_version(void);

@@ -239,3 +239,42 @@ def round_bf16(x: float) -> float:
 
 
 os.environ.setdefault("OMP_PROC_BIND", "false")
+
+
+# ------------------------------------------------------------------ 1-D path
+def _lib1d():
+    L = lib()
+    if not getattr(L, "_adv1d_ready", False):
+        L.oracle_adv1d_run.restype = _I
+        L.oracle_adv1d_run.argtypes = [_P, _I, _I, _P, _P, _I, _D, _I, _P]
+        L.oracle_face_1d.restype = _D
+        L.oracle_face_1d.argtypes = [_P, _I, _I, _P]
+        L.oracle_features_1d.argtypes = [_P, _P]
+        L._adv1d_ready = True
+    return L
+
+
+def adv1d_run(u, velocities, dts, dxinv, scheme="js", mlp=None):
+    """Advance a batch (B, N) of periodic 1-D trajectories with the signed SSPRK3
+    step (rk3_advection.py:82-131) over the dt schedule ``dts``."""
+    u = np.array(np.atleast_2d(u), dtype=np.float64, order="C")
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(velocities, dtype=np.float64), (u.shape[0],)))
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    packed = pack_mlp(mlp, scheme)
+    rc = _lib1d().oracle_adv1d_run(_ptr(u), u.shape[0], u.shape[1], _ptr(a), _ptr(dts), dts.size,
+                                   float(dxinv), SCHEMES[scheme], _ptr(packed))
+    if rc:
+        raise RuntimeError(f"oracle_adv1d_run failed: {rc}")
+    return u
+
+
+def face_1d(q, lr, scheme="js", mlp=None):
+    q = _f64(q)
+    return float(_lib1d().oracle_face_1d(_ptr(q), lr, SCHEMES[scheme], _ptr(pack_mlp(mlp, scheme))))
+
+
+def features_1d(q):
+    q = _f64(q)
+    f = np.zeros(5)
+    _lib1d().oracle_features_1d(_ptr(q), _ptr(f))
+    return f

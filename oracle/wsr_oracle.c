@@ -958,3 +958,220 @@ void oracle_set_num_threads(int n) {
     (void)n;
 #endif
 }
+
+/* ======================================================================== */
+/* 1-D periodic scalar advection (torch path of the reference)              */
+/*   ADV = code/snapshot/teacherfree_lab_weno5_v20_distance_balanced_mlp_f32_fast/rk3_advection.py */
+/*   W5  = code/snapshot/teacherfree_lab_weno5/weno5_core.py                 */
+/*   AP  = code/snapshot/teacherfree_lab_weno5/apost_advect.py               */
+/*   V9  = code/snapshot/teacherfree_lab_weno5_v9_exact_fvm/v9_model.py      */
+/*   V12 = code/snapshot/teacherfree_lab_weno5_v12_reflection_sym/v12_model.py */
+/*   M32 = code/snapshot/teacherfree_lab_weno5_v20_distance_balanced_mlp_f32/v20_mlp_f32_model.py */
+/* ======================================================================== */
+
+/* W5:186-201 weno5_features (scale channel (log10+16)/16), followed by the
+ * model's V9:16-28 remap to (log10+4)/4 as the V12 / M32 models do */
+static void features_1d(const double q[5], double f[5]) {
+    double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+    double d20 = q0 - 2.0 * q1 + q2;
+    double d21 = q1 - 2.0 * q2 + q3;
+    double d22 = q2 - 2.0 * q3 + q4;
+    double delta0 = (13.0 / 12.0) * fabs(d20) + 0.25 * fabs(q0 - 4.0 * q1 + 3.0 * q2);
+    double delta1 = (13.0 / 12.0) * fabs(d21) + 0.25 * fabs(q1 - q3);
+    double delta2 = (13.0 / 12.0) * fabs(d22) + 0.25 * fabs(3.0 * q2 - 4.0 * q3 + q4);
+    double delta_max = dmax(dmax(delta0, delta1), delta2);
+    double inv_delta_max = 1.0 / dmax(delta_max, 1.0e-15);
+    const double eps = 1.0e-15;
+    double g0 = fabs(d20) / (fabs(q1 - q0) + fabs(q2 - q1) + eps);
+    double g1 = fabs(d21) / (fabs(q2 - q1) + fabs(q3 - q2) + eps);
+    double g2 = fabs(d22) / (fabs(q3 - q2) + fabs(q4 - q3) + eps);
+    double gamma_s = dmin(dmax(dmax(dmax(g0, g1), g2), 0.0), 1.0);
+    double q_scale = dmax(dmax(dmax(dmax(fabs(q0), fabs(q1)), fabs(q2)), dmax(fabs(q3), fabs(q4))), 1.0);
+    double relative = dmax(delta_max / q_scale, 1.0e-30);
+    double old_scale = dmin(dmax((log10(relative) + 16.0) / 16.0, 0.0), 1.0);
+    double log10_rel = 16.0 * old_scale - 16.0;
+    double new_scale = dmin(dmax((log10_rel - (-4.0)) / 4.0, 0.0), 1.0);
+    f[0] = delta0 * inv_delta_max;
+    f[1] = delta1 * inv_delta_max;
+    f[2] = delta2 * inv_delta_max;
+    f[3] = gamma_s;
+    f[4] = new_scale;
+}
+
+/* W5:43-55 SharedBadnessMLP.forward: h = x @ w + b (the bias after the product) */
+static void mlp_ratios_torch64(const double x[5], const double *p, double r[3]) {
+    double h1[10], h2[6], h3[6], raw[3];
+    for (int o = 0; o < 10; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 5; ++k) z = z + x[k] * p[OFF_W1 + k * 10 + o];
+        z = z + p[OFF_B1 + o];
+        h1[o] = z * (1.0 / (1.0 + exp(-z)));
+    }
+    for (int o = 0; o < 6; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 10; ++k) z = z + h1[k] * p[OFF_W2 + k * 6 + o];
+        z = z + p[OFF_B2 + o];
+        h2[o] = z * (1.0 / (1.0 + exp(-z)));
+    }
+    for (int o = 0; o < 6; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 6; ++k) z = z + h2[k] * p[OFF_W3 + k * 6 + o];
+        z = z + p[OFF_B3 + o];
+        h3[o] = z * (1.0 / (1.0 + exp(-z)));
+    }
+    for (int o = 0; o < 3; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 6; ++k) z = z + h3[k] * p[OFF_W4 + k * 3 + o];
+        raw[o] = 6.0 * tanh((z + p[OFF_B4 + o]) / 6.0);
+    }
+    double m = dmax(dmax(raw[0], raw[1]), raw[2]);
+    double e0 = exp(raw[0] - m), e1 = exp(raw[1] - m), e2 = exp(raw[2] - m);
+    double s = e0 + e1 + e2;
+    r[0] = e0 / s; r[1] = e1 / s; r[2] = e2 / s;
+}
+
+/* M32 raw_forward: the same in FP32 after the cast of the remapped features */
+static void mlp_ratios_torch32(const float x[5], const float *p, float r[3]) {
+    float h1[10], h2[6], h3[6], raw[3];
+    for (int o = 0; o < 10; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 5; ++k) z = z + x[k] * p[OFF_W1 + k * 10 + o];
+        z = z + p[OFF_B1 + o];
+        h1[o] = z * (1.0f / (1.0f + expf(-z)));
+    }
+    for (int o = 0; o < 6; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 10; ++k) z = z + h1[k] * p[OFF_W2 + k * 6 + o];
+        z = z + p[OFF_B2 + o];
+        h2[o] = z * (1.0f / (1.0f + expf(-z)));
+    }
+    for (int o = 0; o < 6; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 6; ++k) z = z + h2[k] * p[OFF_W3 + k * 6 + o];
+        z = z + p[OFF_B3 + o];
+        h3[o] = z * (1.0f / (1.0f + expf(-z)));
+    }
+    for (int o = 0; o < 3; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 6; ++k) z = z + h3[k] * p[OFF_W4 + k * 3 + o];
+        raw[o] = 6.0f * tanhf((z + p[OFF_B4 + o]) / 6.0f);
+    }
+    float m = fmaxf_(fmaxf_(raw[0], raw[1]), raw[2]);
+    float e0 = expf(raw[0] - m), e1 = expf(raw[1] - m), e2 = expf(raw[2] - m);
+    float s = e0 + e1 + e2;
+    r[0] = e0 / s; r[1] = e1 / s; r[2] = e2 / s;
+}
+
+/* V12:46-50 / M32 forward: 0.5 [M(x) + P3 M(P5 x)] */
+static void sym_ratios_1d(const double f[5], int scheme, const void *mlp, double r[3]) {
+    double fr[5] = {f[2], f[1], f[0], f[3], f[4]};
+    double d[3], e[3];
+    if (scheme == SCHEME_SR64) {
+        mlp_ratios_torch64(f, (const double *)mlp, d);
+        mlp_ratios_torch64(fr, (const double *)mlp, e);
+    } else {
+        float xf[5], xr[5], rd[3], re[3];
+        for (int k = 0; k < 5; ++k) { xf[k] = (float)f[k]; xr[k] = (float)fr[k]; }
+        if (scheme == SCHEME_SR32) {
+            mlp_ratios_torch32(xf, (const float *)mlp, rd);
+            mlp_ratios_torch32(xr, (const float *)mlp, re);
+        } else {
+            mlp_ratios_bf16(xf, (const float *)mlp, rd);
+            mlp_ratios_bf16(xr, (const float *)mlp, re);
+        }
+        for (int k = 0; k < 3; ++k) { d[k] = rd[k]; e[k] = re[k]; }
+    }
+    r[0] = 0.5 * (d[0] + e[2]);
+    r[1] = 0.5 * (d[1] + e[1]);
+    r[2] = 0.5 * (d[2] + e[0]);
+}
+
+/* W5:133-137 plateau_mask */
+static int plateau_1d(const double q[5]) {
+    double raw[4];
+    raw_sensors(q[0], q[1], q[2], q[3], q[4], raw);
+    double delta_max = dmax(dmax(raw[0], raw[1]), raw[2]);
+    double q_scale = dmax(dmax(dmax(dmax(fabs(q[0]), fabs(q[1])), fabs(q[2])), dmax(fabs(q[3]), fabs(q[4]))), 1.0);
+    return delta_max <= 1.0e-13 * q_scale;
+}
+
+/* one face value of the 1-D scheme: head 0 (lr=1, x_{i+1/2} of cell i) or 1 (lr=2) */
+static double face_1d(const double q[5], int lr, int scheme, const void *mlp) {
+    int head = lr == 1 ? HEAD_NORMAL_LR1 : HEAD_NORMAL_LR2;
+    double s[3], d[3], w[3];
+    candidates(q, head, s);
+    optimal_weights(head, d);
+    if (plateau_1d(q)) {
+        w[0] = d[0]; w[1] = d[1]; w[2] = d[2];
+    } else if (scheme == SCHEME_JS) {
+        /* W5:235-239 classical_omega: d / (beta + 1e-6)^2, normalised */
+        double b[3];
+        js_beta(q, b);
+        double a0 = d[0] / ((b[0] + 1.0e-6) * (b[0] + 1.0e-6));
+        double a1 = d[1] / ((b[1] + 1.0e-6) * (b[1] + 1.0e-6));
+        double a2 = d[2] / ((b[2] + 1.0e-6) * (b[2] + 1.0e-6));
+        double sum = a0 + a1 + a2;
+        w[0] = a0 / sum; w[1] = a1 / sum; w[2] = a2 / sum;
+    } else {
+        /* W5:220-224 omega_from_ratio: beta = 3 r, alpha = d / (beta + 1e-12)^2 */
+        double f[5], r[3];
+        features_1d(q, f);
+        sym_ratios_1d(f, scheme, mlp, r);
+        double a0 = d[0] / ((3.0 * r[0] + 1.0e-12) * (3.0 * r[0] + 1.0e-12));
+        double a1 = d[1] / ((3.0 * r[1] + 1.0e-12) * (3.0 * r[1] + 1.0e-12));
+        double a2 = d[2] / ((3.0 * r[2] + 1.0e-12) * (3.0 * r[2] + 1.0e-12));
+        double sum = a0 + a1 + a2;
+        w[0] = a0 / sum; w[1] = a1 / sum; w[2] = a2 / sum;
+    }
+    return w[0] * s[0] + w[1] * s[1] + w[2] * s[2];
+}
+
+/* ADV:82-115 rhs_signed for one trajectory; face i is x_{i+1/2} */
+static void rhs_1d(const double *u, int n, double a, double dxinv, int scheme, const void *mlp,
+                   double *face, double *rhs) {
+    for (int i = 0; i < n; ++i) {
+        double q[5];
+        if (a > 0.0) {
+            for (int m = 0; m < 5; ++m) q[m] = u[((i - 2 + m) % n + n) % n];
+            face[i] = a * face_1d(q, 1, scheme, mlp);
+        } else {
+            for (int m = 0; m < 5; ++m) q[m] = u[((i - 1 + m) % n + n) % n];
+            face[i] = a * face_1d(q, 2, scheme, mlp);
+        }
+    }
+    for (int i = 0; i < n; ++i) rhs[i] = -(face[i] - face[(i - 1 + n) % n]) * dxinv;
+}
+
+/* ADV:118-131 ssprk3_step_signed (AP:41-45 for a = +1) */
+static void ssprk3_1d(double *u, int n, double a, double dt, double dxinv, int scheme, const void *mlp,
+                      double *scratch) {
+    double *face = scratch, *rhs = scratch + n, *u1 = scratch + 2 * n, *u2 = scratch + 3 * n;
+    rhs_1d(u, n, a, dxinv, scheme, mlp, face, rhs);
+    for (int i = 0; i < n; ++i) u1[i] = u[i] + dt * rhs[i];
+    rhs_1d(u1, n, a, dxinv, scheme, mlp, face, rhs);
+    for (int i = 0; i < n; ++i) u2[i] = 0.75 * u[i] + 0.25 * (u1[i] + dt * rhs[i]);
+    rhs_1d(u2, n, a, dxinv, scheme, mlp, face, rhs);
+    for (int i = 0; i < n; ++i) u[i] = u[i] / 3.0 + (2.0 / 3.0) * (u2[i] + dt * rhs[i]);
+}
+
+/* batch of trajectories u (batch x n, advanced in place) with velocities a[b] in {-1,+1};
+ * dts[nsteps] is the step schedule; dxinv = 1/dx.  Threads over the batch. */
+int oracle_adv1d_run(double *u, int batch, int n, const double *a, const double *dts, int nsteps,
+                     double dxinv, int scheme, const void *mlp) {
+    if (n < 5 || batch < 1) return -1;
+    if (scheme != SCHEME_JS && scheme != SCHEME_SR64 && scheme != SCHEME_SR32 && scheme != SCHEME_SRBF16)
+        return -2;
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < batch; ++b) {
+        double *scratch = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+        if (!scratch) { rc = -3; continue; }
+        for (int s = 0; s < nsteps; ++s) ssprk3_1d(u + (size_t)b * n, n, a[b], dts[s], dxinv, scheme, mlp, scratch);
+        free(scratch);
+    }
+    return rc;
+}
+
+double oracle_face_1d(const double *q, int lr, int scheme, const void *mlp) { return face_1d(q, lr, scheme, mlp); }
+
+void oracle_features_1d(const double *q, double *f) { features_1d(q, f); }

@@ -1,0 +1,105 @@
+"""Hand-written adjoint of the 1-D rollout (wsr_adv1d_train_forward / _adjoint).
+
+Checked against (1) the reference's own torch-autograd gradients of the same loss
+(tests/golden/grad_fixture.npz, written by tests/golden/make_grad_fixture.py from
+the reference's rk3_advection / model / v12 modules), (2) central finite
+differences of the CPU oracle's FP64 forward, and (3) the chain rule on the state
+(adjoint of the initial state vs finite differences).
+Tolerances: FP64 closure 1e-8 relative (max-norm over the 189 gradients); FP32
+closure 2e-3 relative (FP32 MLP with the GPU's exp2 / reciprocal against torch's
+FP32 autograd).
+"""
+from __future__ import annotations
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+FIX = Path(__file__).resolve().parent / "golden" / "grad_fixture.npz"
+NAMES = ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4")
+
+
+def fixture():
+    return np.load(FIX, allow_pickle=False)
+
+
+def flat_grad(fx, prefix):
+    return np.concatenate([fx[f"{prefix}_grad_{n}"].reshape(-1) for n in NAMES])
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_adjoint_matches_reference_autograd(mlp64, mlp32, gpu_device, precision):
+    from weno_sr_publishing_amd.training import ClosureTrainer
+    fx = fixture()
+    mlp = mlp64 if precision == "fp64" else mlp32
+    tr = ClosureTrainer({k: np.asarray(v)[0] for k, v in mlp.items()}, precision, device=gpu_device)
+    loss, grad, final = tr.loss_and_grad(torch.tensor(fx["u0"]), torch.tensor(fx["velocities"]),
+                                         torch.tensor(fx["target"]), fx["dts"])
+    ref = flat_grad(fx, precision)
+    g = grad.cpu().numpy()
+    rel = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
+    fin = np.max(np.abs(final.cpu().numpy() - fx[f"{precision}_final"]))
+    print(f"adjoint {precision}: grad rel {rel:.3e}, final-state diff {fin:.3e}, loss {float(loss)!r} "
+          f"vs {float(fx[f'{precision}_loss'])!r}")
+    if precision == "fp64":
+        assert fin <= 1e-13 and rel <= 1e-8
+        assert abs(float(loss) - float(fx["fp64_loss"])) <= 1e-14
+    else:
+        assert fin <= 1e-6 and rel <= 2e-3
+
+
+def test_adjoint_matches_oracle_finite_differences(oracle, mlp64, gpu_device):
+    """Central differences of the oracle's FP64 loss for 24 parameters and the
+    initial-state adjoint for 6 cells."""
+    from weno_sr_publishing_amd.training import ClosureTrainer, flatten, unflatten
+    fx = fixture()
+    base = {k: np.asarray(v)[0] for k, v in mlp64.items()}
+    tr = ClosureTrainer(base, "fp64", device=gpu_device)
+    _, grad, _, lam0 = tr.loss_and_grad(torch.tensor(fx["u0"]), torch.tensor(fx["velocities"]),
+                                        torch.tensor(fx["target"]), fx["dts"], return_state_grad=True)
+    g = grad.cpu().numpy()
+    flat = flatten(base)
+    n = int(fx["n"])
+
+    def loss_of(f, u0=fx["u0"]):
+        p = {k: v[None, ...] if v.ndim == 2 else v[None, :] for k, v in unflatten(f).items()}
+        fin = oracle.adv1d_run(u0, fx["velocities"], fx["dts"], n, "sr64", p)
+        return float(np.mean((fin - fx["target"]) ** 2))
+
+    rng = np.random.default_rng(2)
+    picks = sorted(set(rng.choice(189, size=24, replace=False).tolist()) | {0, 55, 130, 170, 187})
+    for p in picks:
+        h = 1e-6 * max(1.0, abs(flat[p]))
+        fp, fm = flat.copy(), flat.copy()
+        fp[p] += h
+        fm[p] -= h
+        fd = (loss_of(fp) - loss_of(fm)) / (2 * h)
+        assert abs(fd - g[p]) <= 1e-6 * max(1e-3, np.max(np.abs(g))), (p, fd, g[p])
+    lam = lam0.cpu().numpy()
+    for (b, i) in ((0, 3), (1, 10), (2, 16), (3, 31), (0, 20), (2, 0)):
+        h = 1e-6
+        up, um = fx["u0"].copy(), fx["u0"].copy()
+        up[b, i] += h
+        um[b, i] -= h
+        fd = (loss_of(flat, up) - loss_of(flat, um)) / (2 * h)
+        assert abs(fd - lam[b, i]) <= 1e-6 * max(1e-3, np.max(np.abs(lam))), (b, i, fd, lam[b, i])
+    ref_u = fx["fp64_grad_u0"]
+    assert np.max(np.abs(lam - ref_u)) <= 1e-9 * np.max(np.abs(ref_u))
+
+
+def test_training_step_reduces_loss(gpu_device):
+    """A few Adam steps on a 64-trajectory batch of solver-generated data lower the loss."""
+    from weno_sr_publishing_amd.profiles import make_batch, shifted_target
+    from weno_sr_publishing_amd.training import ClosureTrainer, distance_schedule
+    n = 128
+    u0, vel = make_batch(64, n, seed=7)
+    target = shifted_target(u0, vel, 16)
+    dts = distance_schedule(n, 16.0, 0.6)
+    tr = ClosureTrainer(None, "fp32", lr=3e-3, device=gpu_device, seed=41)
+    losses = [tr.step(torch.tensor(u0), torch.tensor(vel), torch.tensor(target), dts) for _ in range(8)]
+    print("losses", losses)
+    assert np.all(np.isfinite(losses)) and losses[-1] < losses[0]

@@ -312,3 +312,80 @@ def test_quadrant_initial_state():
     sw = prim_to_con(1.1, 0.8939, 0.8939, 1.1)
     np.testing.assert_allclose(u[3 + 30, 3 + 30], ne, rtol=1e-14)
     np.testing.assert_allclose(u[3 + 5, 3 + 5], sw, rtol=1e-14)
+
+
+# --------------------------------------------------------------- 1-D (GSTE)
+GSTE_TOL = {"weno5_js": 1e-12, "weno5_sr_f64": 1e-12, "weno5_sr_f32": 5e-7}
+
+
+@pytest.mark.parametrize("key", ["weno5_js", "weno5_sr_f64", "weno5_sr_f32"])
+def test_gste_metrics_match_reference_table(oracle, mlp64, mlp32, key):
+    """1-D GSTE, N=200, t=10, CFL 0.6 (1667 SSPRK3 steps): L1/L2/Linf/TV of the
+    reference's recorded table (experiments/01_gste_long_time_advection/tables/gste_metrics.csv)."""
+    from weno_sr_publishing_amd.initial import gste_cell_averages
+    rows = {r["method"]: r for r in golden("gste_metrics.json")["rows"]}
+    ref = rows[key]
+    nx, t_end, cfl = int(ref["nx"]), float(ref["t_end"]), float(ref["cfl"])
+    dx = 2.0 / nx
+    steps = int(math.ceil(t_end / (cfl * dx)))
+    _, u0 = gste_cell_averages(nx, 0.0, 15)
+    _, exact = gste_cell_averages(nx, t_end, 15)
+    scheme, mlp = {"weno5_js": ("js", None), "weno5_sr_f64": ("sr64", mlp64), "weno5_sr_f32": ("sr32", mlp32)}[key]
+    final = oracle.adv1d_run(u0[None, :], [1.0], [t_end / steps] * steps, 1.0 / dx, scheme, mlp)[0]
+    d = final - exact
+    got = {"l1": np.mean(np.abs(d)), "l2": np.sqrt(np.mean(d * d)), "linf": np.max(np.abs(d)),
+           "tv": np.sum(np.abs(final - np.roll(final, 1)))}
+    for k, v in got.items():
+        assert abs(v / float(ref[k]) - 1.0) <= GSTE_TOL[key], (k, v, ref[k])
+
+
+def test_adv1d_reflection_symmetry_and_mass(oracle, mlp32):
+    """self_test.py:69-111: periodic mass conservation and bidirectional reflection
+    (flip the profile and a -> -a gives the flipped solution)."""
+    rng = np.random.default_rng(4)
+    n = 64
+    x = (np.arange(n) + 0.5) / n
+    u = np.sin(2 * np.pi * x) + (x > 0.3) * (x < 0.55) + 0.2 * rng.normal(size=n)
+    dts = [0.5 / n] * 20
+    for scheme in ("js", "sr32"):
+        pos = oracle.adv1d_run(u[None, :], [1.0], dts, n, scheme, mlp32)[0]
+        neg = oracle.adv1d_run(u[::-1][None, :], [-1.0], dts, n, scheme, mlp32)[0]
+        assert abs(pos.sum() - u.sum()) < 2e-12 * n
+        assert np.max(np.abs(pos - neg[::-1])) < 2e-12
+
+
+def test_adv1d_rollout_matches_reference_torch(oracle, mlp64, mlp32):
+    """Final states of the reference's own torch rollouts (tests/golden/grad_fixture.npz,
+    rk3_advection.rollout_distance with the FP32 model; v12 ssprk3_step_signed with FP64)."""
+    from pathlib import Path
+    fx = np.load(Path(__file__).resolve().parent / "golden" / "grad_fixture.npz", allow_pickle=False)
+    n = int(fx["n"])
+    f64 = oracle.adv1d_run(fx["u0"], fx["velocities"], fx["dts"], n, "sr64", mlp64)
+    assert np.max(np.abs(f64 - fx["fp64_final"])) <= 1e-14
+    f32 = oracle.adv1d_run(fx["u0"], fx["velocities"], fx["dts"], n, "sr32", mlp32)
+    assert np.max(np.abs(f32 - fx["fp32_final"])) <= 1e-6
+
+
+def test_oracle_fd_gradient_matches_reference_autograd(oracle, mlp64):
+    """Central differences of the oracle's FP64 loss against the reference's autograd
+    gradient for a handful of parameters (pins the fixture and the oracle together)."""
+    from pathlib import Path
+    from weno_sr_publishing_amd.training import flatten, unflatten
+    fx = np.load(Path(__file__).resolve().parent / "golden" / "grad_fixture.npz", allow_pickle=False)
+    n = int(fx["n"])
+    base = flatten({k: np.asarray(v)[0] for k, v in mlp64.items()})
+    ref = np.concatenate([fx[f"fp64_grad_{k}"].reshape(-1) for k in ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4")])
+
+    def loss(f):
+        p = {k: v[None, ...] if v.ndim == 2 else v[None, :] for k, v in unflatten(f).items()}
+        fin = oracle.adv1d_run(fx["u0"], fx["velocities"], fx["dts"], n, "sr64", p)
+        return float(np.mean((fin - fx["target"]) ** 2))
+
+    assert abs(loss(base) - float(fx["fp64_loss"])) <= 1e-14
+    for p in (0, 7, 52, 75, 124, 140, 165, 172, 186, 188):
+        h = 1e-6 * max(1.0, abs(base[p]))
+        fp, fm = base.copy(), base.copy()
+        fp[p] += h
+        fm[p] -= h
+        fd = (loss(fp) - loss(fm)) / (2 * h)
+        assert abs(fd - ref[p]) <= 1e-6 * max(1e-3, np.max(np.abs(ref))), (p, fd, ref[p])

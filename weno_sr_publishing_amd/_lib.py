@@ -19,7 +19,10 @@ ERRORS = {-1: "bad argument", -2: "SR scheme without MLP parameters", -3: "works
 EXPORTS = (
     "wsr_euler2d_workspace_bytes", "wsr_euler2d_stage", "wsr_euler2d_max_speed",
     "wsr_euler2d_loop_init", "wsr_euler2d_loop_steps", "wsr_euler2d_loop_status", "wsr_version",
+    "wsr_euler2d_loop_steps_timed", "wsr_euler2d_counters", "wsr_adv1d_rollout",
+    "wsr_adv1d_train_forward", "wsr_adv1d_train_adjoint",
 )
+FLAG_COUNT_CLOSURE = 1
 
 
 class Euler2DConfig(ctypes.Structure):
@@ -27,10 +30,18 @@ class Euler2DConfig(ctypes.Structure):
     _fields_ = [
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ghost", ctypes.c_int32),
         ("scheme", ctypes.c_int32), ("riemann", ctypes.c_int32), ("boundary", ctypes.c_int32),
-        ("eno_cutoff", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("eno_cutoff", ctypes.c_int32), ("flags", ctypes.c_int32),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("gamma", ctypes.c_double),
         ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
         ("mlp", ctypes.c_void_p),
+    ]
+
+
+class Adv1DConfig(ctypes.Structure):
+    """Mirror of ``wsr_adv1d_config``."""
+    _fields_ = [
+        ("batch", ctypes.c_int32), ("n", ctypes.c_int32), ("scheme", ctypes.c_int32),
+        ("reserved", ctypes.c_int32), ("dxinv", ctypes.c_double), ("mlp", ctypes.c_void_p),
     ]
 
 
@@ -69,6 +80,18 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler2d_loop_status.restype = ctypes.c_int
         L.wsr_euler2d_loop_status.argtypes = [P, ctypes.POINTER(I32), ctypes.POINTER(D),
                                               ctypes.POINTER(D), ctypes.POINTER(I32), P]
+        L.wsr_euler2d_loop_steps_timed.restype = ctypes.c_int
+        L.wsr_euler2d_loop_steps_timed.argtypes = [CFG, P, P, P, P, I32, P, SZ, P, ctypes.POINTER(D),
+                                                   ctypes.POINTER(I32)]
+        L.wsr_euler2d_counters.restype = ctypes.c_int
+        L.wsr_euler2d_counters.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), P]
+        L.wsr_adv1d_rollout.restype = ctypes.c_int
+        L.wsr_adv1d_rollout.argtypes = [ctypes.POINTER(Adv1DConfig), P, P, P, I32, P, P]
+        A1 = ctypes.POINTER(Adv1DConfig)
+        L.wsr_adv1d_train_forward.restype = ctypes.c_int
+        L.wsr_adv1d_train_forward.argtypes = [A1, P, P, P, I32, P, P, P]
+        L.wsr_adv1d_train_adjoint.restype = ctypes.c_int
+        L.wsr_adv1d_train_adjoint.argtypes = [A1, P, P, P, P, I32, P, P, P, P]
         L.wsr_version.restype = ctypes.c_char_p
         _lib = L
     return _lib

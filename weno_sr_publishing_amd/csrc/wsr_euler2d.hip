@@ -47,6 +47,10 @@ struct Ctl {
     double t, dt, t_end, cfl_h, dt_last;
     unsigned long long speed_bits;
     int32_t step, done, nan_flag, max_steps;
+    // closure evaluations (WSR_FLAG_COUNT_CLOSURE): [0] normal reconstructions,
+    // [1] of which evaluated the closure (not a plateau), [2] Gauss-pair
+    // reconstructions, [3] of which evaluated the closure
+    unsigned long long counters[4];
 };
 
 struct Grid {
@@ -123,8 +127,15 @@ struct SweepArgs {
     Grid g;
     double dt_host;
     const Ctl *ctl;
+    unsigned long long *counters;   // nullptr unless closure evaluations are counted
     int eno;
 };
+
+// wave-level count of lanes with `flag` set, added to *dst by one lane
+__device__ __forceinline__ void count_lanes(unsigned long long *dst, bool flag) {
+    unsigned long long m = __ballot(flag);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(dst, (unsigned long long)__popcll(m));
+}
 
 template <int DIR>
 __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
@@ -190,7 +201,12 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
             const double *q = T + ((fl + s + m) * NT1 + tau) * 4;
             c[m] = (l[0] * q[0] + l[1] * q[1] + l[2] * q[2] + l[3] * q[3]) * scale;
         }
-        double val = recon_normal<SCHEME>(c, s == 0, mlp, A.eno, h_n);
+        bool evaluated;
+        double val = recon_normal<SCHEME>(c, s == 0, mlp, A.eno, h_n, evaluated);
+        if (A.counters) {
+            count_lanes(A.counters + 0, valid);
+            count_lanes(A.counters + 1, valid && evaluated);
+        }
         const int qb = lane & ~3;
         double cf0 = __shfl(val, qb + 0), cf1 = __shfl(val, qb + 1);
         double cf2 = __shfl(val, qb + 2), cf3 = __shfl(val, qb + 3);
@@ -212,7 +228,13 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
 #pragma unroll
         for (int m = 0; m < 5; ++m) q[m] = FV[((tl + m) * TN + fl) * 8 + gi];
         double g1, g2;
-        recon_gauss<SCHEME>(q, mlp, A.eno, h_t, g1, g2);
+        bool evaluated;
+        recon_gauss<SCHEME>(q, mlp, A.eno, h_t, g1, g2, evaluated);
+        if (A.counters) {
+            const bool owned = valid && (f0 + fl) < nfaces && (t0 + tl) < ntrans;
+            count_lanes(A.counters + 2, owned);
+            count_lanes(A.counters + 3, owned && evaluated);
+        }
         // lanes with s = 0 evaluate the flux at Gauss point 1, s = 1 at point 2
         const int gb = lane & ~7;
         double ul[4], ur[4];
@@ -332,6 +354,7 @@ __global__ void k_reset_speed(Ctl *ctl) {
 __global__ void k_loop_init(Ctl *ctl, double t_end, double cfl_h, int max_steps) {
     ctl->t = 0.0; ctl->dt = 0.0; ctl->t_end = t_end; ctl->cfl_h = cfl_h; ctl->dt_last = 0.0;
     ctl->speed_bits = 0ull; ctl->step = 0; ctl->done = 0; ctl->nan_flag = 0; ctl->max_steps = max_steps;
+    for (int i = 0; i < 4; ++i) ctl->counters[i] = 0ull;
 }
 
 // dt of the next step (warp_weno5_helpers.py:189-207 and the loop head of run_to_time)
@@ -428,9 +451,16 @@ static void launch_sweep_t(const double *bc, double *F, const SweepArgs &A, cons
     hipLaunchKernelGGL((k_sweep<DIR, SCHEME, RIEMANN>), grid, dim3(BLOCK), 0, st, bc, F, A, mlp);
 }
 
+// optional per-launch timing of the sweep kernels (HIP events on the launch stream)
+struct SweepTimer {
+    hipEvent_t ev[2 * 6 * 64];
+    int used = 0, cap = 0;
+};
+
 template <int DIR>
 static void launch_sweep(int scheme, int riemann, const double *bc, double *F, const SweepArgs &A,
-                         const void *mlp_host, hipStream_t st) {
+                         const void *mlp_host, hipStream_t st, SweepTimer *timer = nullptr) {
+    if (timer && timer->used + 2 <= timer->cap) (void)hipEventRecord(timer->ev[timer->used], st);
 #define WSR_CASE(S)                                                                              \
     case S:                                                                                      \
         if (riemann == RIEMANN_FORCE) launch_sweep_t<DIR, S, RIEMANN_FORCE>(bc, F, A, mlp_host, st); \
@@ -444,12 +474,16 @@ static void launch_sweep(int scheme, int riemann, const double *bc, double *F, c
         WSR_CASE(SCHEME_Z)
     }
 #undef WSR_CASE
+    if (timer && timer->used + 2 <= timer->cap) {
+        (void)hipEventRecord(timer->ev[timer->used + 1], st);
+        timer->used += 2;
+    }
 }
 
 // one stage; ctl != nullptr selects the device-resident dt / time
 static void enqueue_stage(const wsr_euler2d_config *cfg, const Workspace &w, const double *u_stage,
                           const double *u0, double *u_out, double dt, int rk, double boundary_time,
-                          Ctl *ctl, int fold_speed, double *dts, hipStream_t st) {
+                          Ctl *ctl, int fold_speed, double *dts, hipStream_t st, SweepTimer *timer = nullptr) {
     Grid g = make_grid(cfg);
     const int ncells_p = g.nxt * g.nyt;
     double frac = rk == 1 ? 0.0 : (rk == 2 ? 1.0 : 0.5);
@@ -457,8 +491,9 @@ static void enqueue_stage(const wsr_euler2d_config *cfg, const Workspace &w, con
                        cfg->boundary, boundary_time, (const Ctl *)ctl, frac);
     SweepArgs A;
     A.g = g; A.dt_host = dt; A.ctl = ctl; A.eno = cfg->eno_cutoff;
-    launch_sweep<1>(cfg->scheme, cfg->riemann, w.bc, w.Fx, A, cfg->mlp, st);
-    launch_sweep<2>(cfg->scheme, cfg->riemann, w.bc, w.Fy, A, cfg->mlp, st);
+    A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
+    launch_sweep<1>(cfg->scheme, cfg->riemann, w.bc, w.Fx, A, cfg->mlp, st, timer);
+    launch_sweep<2>(cfg->scheme, cfg->riemann, w.bc, w.Fy, A, cfg->mlp, st, timer);
     const int ncells = g.nx * g.ny;
     hipLaunchKernelGGL(k_update, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const double *)w.bc, u0,
                        u_out, (const double *)w.Fx, (const double *)w.Fy, g, rk, ctl, fold_speed, dts);
@@ -543,6 +578,58 @@ int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1,
         enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st);
     }
     return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, double *u1, double *u2, double *dts,
+                                 int32_t n_steps, void *workspace, size_t workspace_bytes, void *stream,
+                                 double *sweep_ms, int32_t *sweep_launches) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (!u || !u1 || !u2 || !dts || !workspace || n_steps < 0 || n_steps > 64 || !sweep_ms || !sweep_launches)
+        return WSR_E_ARG;
+    if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
+    Workspace w;
+    ws_layout(cfg->nx, cfg->ny, &w, workspace);
+    hipStream_t st = (hipStream_t)stream;
+    SweepTimer timer;
+    timer.cap = 6 * 2 * n_steps;
+    for (int i = 0; i < timer.cap; ++i)
+        if (hipEventCreate(&timer.ev[i]) != hipSuccess) return WSR_E_HIP;
+    for (int n = 0; n < n_steps; ++n) {
+        hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
+        enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st, &timer);
+        enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st, &timer);
+        enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st, &timer);
+    }
+    rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) rc = WSR_E_HIP;
+    // launches after the loop stopped return at once; count only the ones that did work
+    Ctl host;
+    if (hipMemcpy(&host, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost) != hipSuccess) rc = WSR_E_HIP;
+    double total = 0.0;
+    int launches = 0;
+    for (int i = 0; i + 1 < timer.used; i += 2) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, timer.ev[i], timer.ev[i + 1]) == hipSuccess && ms > 0.002f) {
+            total += ms;
+            launches += 1;
+        }
+    }
+    for (int i = 0; i < timer.cap; ++i) (void)hipEventDestroy(timer.ev[i]);
+    *sweep_ms = total;
+    *sweep_launches = launches;
+    return rc;
+}
+
+int wsr_euler2d_counters(void *workspace, uint64_t *out4, void *stream) {
+    if (!workspace || !out4) return WSR_E_ARG;
+    Ctl *ctl = (Ctl *)workspace;
+    hipStream_t st = (hipStream_t)stream;
+    Ctl host;
+    if (hipMemcpyAsync(&host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st) != hipSuccess) return WSR_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return WSR_E_HIP;
+    for (int i = 0; i < 4; ++i) out4[i] = host.counters[i];
+    return WSR_OK;
 }
 
 int wsr_euler2d_loop_status(void *workspace, int32_t *steps, double *t, double *dt_last, int32_t *done,

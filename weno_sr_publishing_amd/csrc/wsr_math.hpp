@@ -449,8 +449,10 @@ __device__ __forceinline__ double sr_combine(const double s[3], const double d[3
 // ------------------------------------------------ scalar reconstructions
 // Normal face value (lr1: right face x_{i+1/2}; otherwise left face x_{i-1/2}).
 template <int SCHEME, class MLP>
-__device__ __forceinline__ double recon_normal(const double q[5], bool lr1, const MLP &P, int eno, double h) {
+__device__ __forceinline__ double recon_normal(const double q[5], bool lr1, const MLP &P, int eno, double h,
+                                               bool &closure_evaluated) {
     double s[3], d[3];
+    closure_evaluated = true;
     candidates_normal(q, lr1, s);
     d_normal(lr1, d);
     if constexpr (SCHEME == SCHEME_JS) {
@@ -467,6 +469,7 @@ __device__ __forceinline__ double recon_normal(const double q[5], bool lr1, cons
     } else {
         double f[5], ib2[3] = {0.0, 0.0, 0.0};
         bool plateau = sr_features(q, f);
+        closure_evaluated = !plateau;
         if (!plateau) {
             double br[3];
             sr_ratios<SCHEME>(f, P, br);
@@ -480,8 +483,9 @@ __device__ __forceinline__ double recon_normal(const double q[5], bool lr1, cons
 // ratios) is evaluated once and shared by the two points.
 template <int SCHEME, class MLP>
 __device__ __forceinline__ void recon_gauss(const double q[5], const MLP &P, int eno, double h,
-                                            double &g1, double &g2) {
+                                            double &g1, double &g2, bool &closure_evaluated) {
     double s1[3], s2[3], d1[3], d2[3];
+    closure_evaluated = true;
     candidates_gauss(q, s1, s2);
     d_gauss(d1, d2);
     if constexpr (SCHEME == SCHEME_JS) {
@@ -500,6 +504,7 @@ __device__ __forceinline__ void recon_gauss(const double q[5], const MLP &P, int
     } else {
         double f[5], ib2[3] = {0.0, 0.0, 0.0};
         bool plateau = sr_features(q, f);
+        closure_evaluated = !plateau;
         if (!plateau) {
             double br[3];
             sr_ratios<SCHEME>(f, P, br);

@@ -72,7 +72,7 @@ class Euler2DSolver:
         self.cfg = _lib.Euler2DConfig(
             nx=params.nx, ny=params.ny, ghost=params.ghost,
             scheme=_lib.SCHEMES[scheme], riemann=_lib.RIEMANN[riemann],
-            boundary=_lib.BOUNDARY[boundary], eno_cutoff=1 if eno_cutoff else 0, reserved=0,
+            boundary=_lib.BOUNDARY[boundary], eno_cutoff=1 if eno_cutoff else 0, flags=0,
             dx=params.dx, dy=params.dy, gamma=params.gamma, cfl=params.cfl, t_end=params.t_end,
             mlp=self._mlp_host.ctypes.data if self._mlp_host is not None else None)
         self._u1 = None
@@ -143,6 +143,26 @@ class Euler2DSolver:
                                               self._u2.data_ptr(), self._dts.data_ptr(), int(n),
                                               self.workspace.data_ptr(), self.ws_bytes, self._stream())
         _lib.check(rc, "wsr_euler2d_loop_steps")
+
+    def loop_steps_timed(self, u: torch.Tensor, n: int) -> tuple[float, int]:
+        """loop_steps with HIP events around every sweep launch; returns (ms, launches)."""
+        ms = ctypes.c_double(0.0)
+        launches = ctypes.c_int32(0)
+        rc = self._lib.wsr_euler2d_loop_steps_timed(
+            ctypes.byref(self.cfg), u.data_ptr(), self._u1.data_ptr(), self._u2.data_ptr(),
+            self._dts.data_ptr(), int(n), self.workspace.data_ptr(), self.ws_bytes, self._stream(),
+            ctypes.byref(ms), ctypes.byref(launches))
+        _lib.check(rc, "wsr_euler2d_loop_steps_timed")
+        return float(ms.value), int(launches.value)
+
+    def set_counting(self, on: bool) -> None:
+        self.cfg.flags = (self.cfg.flags | _lib.FLAG_COUNT_CLOSURE) if on else (self.cfg.flags & ~_lib.FLAG_COUNT_CLOSURE)
+
+    def counters(self) -> list[int]:
+        out = (ctypes.c_uint64 * 4)()
+        rc = self._lib.wsr_euler2d_counters(self.workspace.data_ptr(), out, self._stream())
+        _lib.check(rc, "wsr_euler2d_counters")
+        return [int(v) for v in out]
 
     def loop_status(self) -> tuple[int, float, float, bool]:
         steps = ctypes.c_int32(0)

@@ -129,3 +129,49 @@ def make_sod_state(params, axis: str = "x") -> np.ndarray:
                                            params.gamma, axis) for i in range(nx_total)])
         u[:, :, :] = con[None, :, :]
     return u
+
+
+# ---------------------------------------------------------------- 1-D GSTE
+# Gaussian / square / triangle / semi-ellipse profile on [-1, 1]
+# (code/snapshot/for_paper_results/run_gste.py:38-100).
+GSTE_X_MIN = -1.0
+GSTE_X_MAX = 1.0
+GSTE_LENGTH = GSTE_X_MAX - GSTE_X_MIN
+
+
+def _gste_wrap(x: np.ndarray) -> np.ndarray:
+    return GSTE_X_MIN + np.mod(x - GSTE_X_MIN, GSTE_LENGTH)
+
+
+def gste_point(x) -> np.ndarray:
+    x = _gste_wrap(np.asarray(x, dtype=np.float64))
+    delta = 0.005
+    beta = np.log(2.0) / (36.0 * delta * delta)
+    z, a, alpha = -0.7, 0.5, 10.0
+
+    def gaussian(center):
+        return np.exp(-beta * (x - center) ** 2)
+
+    def ellipse(center):
+        return np.sqrt(np.maximum(1.0 - alpha * alpha * (x - center) ** 2, 0.0))
+
+    out = np.zeros_like(x)
+    mask = (-0.8 < x) & (x < -0.6)
+    out[mask] = (gaussian(z - delta)[mask] + 4.0 * gaussian(z)[mask] + gaussian(z + delta)[mask]) / 6.0
+    mask = (-0.4 < x) & (x < -0.2)
+    out[mask] = 1.0
+    mask = (0.0 < x) & (x < 0.2)
+    out[mask] = 1.0 - np.abs(10.0 * (x[mask] - 0.1))
+    mask = (0.4 < x) & (x < 0.6)
+    out[mask] = (ellipse(a - delta)[mask] + 4.0 * ellipse(a)[mask] + ellipse(a + delta)[mask]) / 6.0
+    return out
+
+
+def gste_cell_averages(nx: int, time: float, quadrature: int = 15) -> tuple[np.ndarray, np.ndarray]:
+    """Cell centres and Gauss-Legendre cell averages of the profile advected by ``time``."""
+    dx = GSTE_LENGTH / nx
+    centers = GSTE_X_MIN + (np.arange(nx) + 0.5) * dx
+    xi, weights = np.polynomial.legendre.leggauss(quadrature)
+    samples = centers[:, None] + 0.5 * dx * xi[None, :] - time
+    averages = 0.5 * np.sum(weights[None, :] * gste_point(samples), axis=1)
+    return centers, averages
