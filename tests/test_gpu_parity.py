@@ -135,15 +135,18 @@ def test_max_speed_and_dt(oracle, gpu_device):
     g = s.new_state(us)
     assert s.max_speed(g) == oracle.max_speed2d(us, nx, ny, GAMMA)
     assert s.compute_dt(g) == oracle.dt2d(us, nx, ny, p.dx, p.dy, GAMMA, p.cfl)
-    # NaN momentum gives a NaN speed and np.max propagates it; a NaN density or
-    # energy is floored by max(., 1e-15) first (warp_weno5_helpers.py:886-894)
+    # The kernel's max(a, b) = a > b ? a : b (warp_weno5_helpers.py:886-896 uses wp.max):
+    # a NaN density or energy is floored to 1e-15 and one NaN velocity loses against the
+    # other one, so only a cell with both momenta NaN gives a NaN speed, which np.max
+    # then propagates and the loop stops.
     bad = us.copy()
-    bad[10, 10, 1] = np.nan
+    bad[10, 10, 1] = bad[10, 10, 2] = np.nan
     assert np.isnan(s.max_speed(s.new_state(bad)))
     assert np.isnan(oracle.max_speed2d(bad, nx, ny, GAMMA))
-    bad = us.copy()
-    bad[10, 10, 0] = np.nan
-    assert s.max_speed(s.new_state(bad)) == oracle.max_speed2d(bad, nx, ny, GAMMA)
+    for comp in (0, 1, 3):
+        bad = us.copy()
+        bad[10, 10, comp] = np.nan
+        assert s.max_speed(s.new_state(bad)) == oracle.max_speed2d(bad, nx, ny, GAMMA)
 
 
 def test_loop_stops_on_nan(gpu_device):
@@ -151,7 +154,7 @@ def test_loop_stops_on_nan(gpu_device):
     nx, ny = 16, 16
     p = Params(nx=nx, ny=ny, x_length=1.0, y_length=1.0, cfl=0.4, t_end=0.1)
     u0 = make_quadrant_initial(nx, ny, "c4")
-    u0[8, 8, 1] = np.nan
+    u0[8, 8, 1] = u0[8, 8, 2] = np.nan
     s = solver(p, "js", None)
     u = s.new_state(u0)
     dts, steps, t = s.run(u, p.t_end)

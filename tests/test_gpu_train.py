@@ -80,15 +80,17 @@ def test_adjoint_matches_oracle_finite_differences(oracle, mlp64, gpu_device):
         fd = (loss_of(fp) - loss_of(fm)) / (2 * h)
         assert abs(fd - g[p]) <= 1e-6 * max(1e-3, np.max(np.abs(g))), (p, fd, g[p])
     lam = lam0.cpu().numpy()
-    for (b, i) in ((0, 3), (1, 10), (2, 16), (3, 31), (0, 20), (2, 0)):
+    ref_u = fx["fp64_grad_u0"]
+    assert np.max(np.abs(lam - ref_u)) <= 1e-9 * np.max(np.abs(ref_u))
+    # finite differences only where every stencil around the cell is smooth: a
+    # perturbation inside a constant region switches the plateau branch on or off
+    for (b, i) in ((0, 3), (1, 10), (0, 20), (1, 25), (1, 4)):
         h = 1e-6
         up, um = fx["u0"].copy(), fx["u0"].copy()
         up[b, i] += h
         um[b, i] -= h
         fd = (loss_of(flat, up) - loss_of(flat, um)) / (2 * h)
         assert abs(fd - lam[b, i]) <= 1e-6 * max(1e-3, np.max(np.abs(lam))), (b, i, fd, lam[b, i])
-    ref_u = fx["fp64_grad_u0"]
-    assert np.max(np.abs(lam - ref_u)) <= 1e-9 * np.max(np.abs(ref_u))
 
 
 def test_training_step_reduces_loss(gpu_device):

@@ -143,6 +143,19 @@ __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
     else { tau = it % ntau; fl = it / ntau; }            // columns fastest (y: contiguous cells)
 }
 
+// LDS plan of one sweep tile (doubles):
+//   FV  [NT1][TN][8]        face values, phase 1 -> phase 2
+//   REG phase 1: T [NN][NT1][4] stencil cells + EIG [N1][4] Roe eigenvalues
+//       phase 2: GV [CH][2][8] Gauss-point values + FL [CH][2][4] fluxes (one chunk)
+constexpr int N1 = NT1 * TN;     // face-lines of phase 1 (256: one per thread)
+constexpr int N2 = TT * TN;      // flux faces of phase 2
+constexpr int CH = BLOCK / 2;    // phase-2 chunk: one (face, Gauss point) flux per thread
+constexpr int FV_SIZE = NT1 * TN * 8;
+constexpr int P1_SIZE = NN * NT1 * 4 + N1 * 4;
+constexpr int P2_SIZE = CH * 16 + CH * 8;
+constexpr int REG_SIZE = P1_SIZE > P2_SIZE ? P1_SIZE : P2_SIZE;
+static_assert(N1 == BLOCK, "phase 1a assigns one face-line per thread");
+
 template <int DIR, int SCHEME, int RIEMANN>
 __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
                                                  SweepArgs A, typename MlpFor<SCHEME>::type mlp) {
@@ -160,8 +173,12 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
     const int f0 = blockIdx.x * TN;
     const int t0 = blockIdx.y * TT;
 
-    __shared__ double T[NN * NT1 * 4];      // stencil tile [n][tau][c]
-    __shared__ double FV[NT1 * TN * 8];     // face values [tau][face][side*4+k]
+    __shared__ double smem[FV_SIZE + REG_SIZE];
+    double *FV = smem;
+    double *T = smem + FV_SIZE;               // phase 1
+    double *EIG = T + NN * NT1 * 4;
+    double *GV = smem + FV_SIZE;              // phase 2 (reuses the phase-1 region)
+    double *FL = GV + CH * 16;
 
     // cooperative tile load, contiguous along the padded row
     for (int e = threadIdx.x; e < NN * NT1; e += BLOCK) {
@@ -177,22 +194,30 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
     }
     __syncthreads();
 
+    // ---------------- phase 1a: Roe average and eigenvalues, one face-line per thread
+    {
+        int fl, tau;
+        item_map<DIR>(threadIdx.x, NT1, fl, tau);
+        Eigen E = roe_eigen(T + ((fl + 2) * NT1 + tau) * 4, T + ((fl + 3) * NT1 + tau) * 4, g.gamma);
+        double *e = EIG + threadIdx.x * 4;
+        e[0] = E.u; e[1] = E.v; e[2] = E.a; e[3] = E.h;
+    }
+    __syncthreads();
+
     const int lane = threadIdx.x;
     const int gi = lane & 7;
     const int s = gi >> 2;           // 0: state left of the face (right face of cell f-1), 1: right state
     const int k = gi & 3;            // characteristic / conserved component
     const int slot = lane >> 3;      // 32 face items per pass
 
-    // ---------------- phase 1: normal characteristic reconstruction
-    constexpr int N1 = NT1 * TN;
+    // ---------------- phase 1b: characteristic projection, WENO face value, back-projection
     for (int base = 0; base < N1; base += BLOCK / 8) {
-        int it = base + slot;
-        const bool valid = it < N1;
+        const int it = base + slot;
         int fl, tau;
-        item_map<DIR>(valid ? it : N1 - 1, NT1, fl, tau);
-        const double *colL = T + ((fl + 2) * NT1 + tau) * 4;
-        const double *colR = T + ((fl + 3) * NT1 + tau) * 4;
-        Eigen E = roe_eigen(colL, colR, g.gamma);
+        item_map<DIR>(it, NT1, fl, tau);
+        const double *e = EIG + it * 4;
+        Eigen E;
+        E.u = e[0]; E.v = e[1]; E.a = e[2]; E.h = e[3];
         double l[4], scale;
         left_row<DIR>(k, E, g.gamma, l, scale);
         double c[5];
@@ -204,58 +229,70 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
         bool evaluated;
         double val = recon_normal<SCHEME>(c, s == 0, mlp, A.eno, h_n, evaluated);
         if (A.counters) {
-            count_lanes(A.counters + 0, valid);
-            count_lanes(A.counters + 1, valid && evaluated);
+            count_lanes(A.counters + 0, true);
+            count_lanes(A.counters + 1, evaluated);
         }
         const int qb = lane & ~3;
         double cf0 = __shfl(val, qb + 0), cf1 = __shfl(val, qb + 1);
         double cf2 = __shfl(val, qb + 2), cf3 = __shfl(val, qb + 3);
         double r[4];
         right_row<DIR>(k, E, r);
-        double out = r[0] * cf0 + r[1] * cf1 + r[2] * cf2 + r[3] * cf3;
-        if (valid) FV[(tau * TN + fl) * 8 + gi] = out;
+        FV[(tau * TN + fl) * 8 + gi] = r[0] * cf0 + r[1] * cf1 + r[2] * cf2 + r[3] * cf3;
     }
     __syncthreads();
 
-    // ---------------- phase 2: transverse Gauss reconstruction + Riemann flux
-    constexpr int N2 = TT * TN;
-    for (int base = 0; base < N2; base += BLOCK / 8) {
-        int it = base + slot;
-        const bool valid = it < N2;
-        int fl, tl;
-        item_map<DIR>(valid ? it : N2 - 1, TT, fl, tl);
-        double q[5];
+    // ---------------- phase 2: transverse Gauss reconstruction, then the Riemann fluxes
+    for (int c0 = 0; c0 < N2; c0 += CH) {
+        // 2a: both Gauss-point values of each (face, side, component), 8 lanes per face
+        for (int base = 0; base < CH; base += BLOCK / 8) {
+            const int ci = base + slot;
+            const int it = c0 + ci;
+            const bool valid = it < N2;
+            int fl, tl;
+            item_map<DIR>(valid ? it : N2 - 1, TT, fl, tl);
+            double q[5];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) q[m] = FV[((tl + m) * TN + fl) * 8 + gi];
-        double g1, g2;
-        bool evaluated;
-        recon_gauss<SCHEME>(q, mlp, A.eno, h_t, g1, g2, evaluated);
-        if (A.counters) {
-            const bool owned = valid && (f0 + fl) < nfaces && (t0 + tl) < ntrans;
-            count_lanes(A.counters + 2, owned);
-            count_lanes(A.counters + 3, owned && evaluated);
+            for (int m = 0; m < 5; ++m) q[m] = FV[((tl + m) * TN + fl) * 8 + gi];
+            double g1, g2;
+            bool evaluated;
+            recon_gauss<SCHEME>(q, mlp, A.eno, h_t, g1, g2, evaluated);
+            if (A.counters) {
+                const bool owned = valid && (f0 + fl) < nfaces && (t0 + tl) < ntrans;
+                count_lanes(A.counters + 2, owned);
+                count_lanes(A.counters + 3, owned && evaluated);
+            }
+            GV[ci * 16 + gi] = g1;
+            GV[ci * 16 + 8 + gi] = g2;
         }
-        // lanes with s = 0 evaluate the flux at Gauss point 1, s = 1 at point 2
-        const int gb = lane & ~7;
-        double ul[4], ur[4];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            double a1 = __shfl(g1, gb + m), a2 = __shfl(g2, gb + m);
-            double b1 = __shfl(g1, gb + 4 + m), b2 = __shfl(g2, gb + 4 + m);
-            ul[m] = s == 0 ? a1 : a2;
-            ur[m] = s == 0 ? b1 : b2;
+        __syncthreads();
+        // 2b: one flux per thread: (face ci, Gauss point gp)
+        {
+            const int ci = threadIdx.x >> 1, gp = threadIdx.x & 1;
+            const double *v = GV + ci * 16 + gp * 8;
+            double ul[4] = {v[0], v[1], v[2], v[3]};
+            double ur[4] = {v[4], v[5], v[6], v[7]};
+            double f[4];
+            if constexpr (RIEMANN == RIEMANN_FORCE) force_flux<DIR>(ul, ur, dt_h, g.gamma, f);
+            else hllc_flux<DIR>(ul, ur, g.gamma, f);
+            double *o = FL + ci * 8 + gp * 4;
+            o[0] = f[0]; o[1] = f[1]; o[2] = f[2]; o[3] = f[3];
         }
-        double f[4];
-        if constexpr (RIEMANN == RIEMANN_FORCE) force_flux<DIR>(ul, ur, dt_h, g.gamma, f);
-        else hllc_flux<DIR>(ul, ur, g.gamma, f);
-        double fk = k == 0 ? f[0] : k == 1 ? f[1] : k == 2 ? f[2] : f[3];
-        double other = __shfl_xor(fk, 4);
-        const int face = f0 + fl, tt = t0 + tl;
-        if (valid && s == 0 && face < nfaces && tt < ntrans) {
-            size_t o = DIR == 1 ? ((size_t)tt * (g.nx + 1) + face) * 4 + k
-                                : ((size_t)face * g.nx + tt) * 4 + k;
-            F[o] = fk * dt_h + other * dt_h;
+        __syncthreads();
+        // 2c: (F(g1) + F(g2)) dt/h, coalesced along the padded row
+        for (int e = threadIdx.x; e < CH * 4; e += BLOCK) {
+            const int ci = e >> 2, kk = e & 3;
+            const int it = c0 + ci;
+            if (it >= N2) continue;
+            int fl, tl;
+            item_map<DIR>(it, TT, fl, tl);
+            const int face = f0 + fl, tt = t0 + tl;
+            if (face < nfaces && tt < ntrans) {
+                size_t o = DIR == 1 ? ((size_t)tt * (g.nx + 1) + face) * 4 + kk
+                                    : ((size_t)face * g.nx + tt) * 4 + kk;
+                F[o] = FL[ci * 8 + kk] * dt_h + FL[ci * 8 + 4 + kk] * dt_h;
+            }
         }
+        __syncthreads();
     }
 }
 

@@ -344,11 +344,16 @@ __device__ __forceinline__ void mlp_raw64(const double x[5], const Mlp64 &P, dou
     r[0] = e0 * inv; r[1] = e1 * inv; r[2] = e2 * inv;
 }
 
-// FP32 closure on the VALU: v_exp_f32 / v_rcp_f32 for the swish and the
-// softmax, tanhf for the bounded logits.
+// FP32 closure on the VALU: v_exp_f32 / v_rcp_f32 for the swish, the bounded
+// logits and the softmax.
 __device__ __forceinline__ float exp_fast(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
 __device__ __forceinline__ float rcp_fast(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float swish32(float x) { return x * rcp_fast(1.0f + exp_fast(-x)); }
+// tanh(y) = sign(y) (1 - e) / (1 + e), e = exp(-2|y|): absolute error ~1e-7
+__device__ __forceinline__ float tanh_fast(float y) {
+    float e = exp_fast(-2.0f * fabsf(y));
+    return copysignf((1.0f - e) * rcp_fast(1.0f + e), y);
+}
 
 __device__ __forceinline__ float round_bf16(float x) {
     uint32_t u = __float_as_uint(x);
@@ -390,7 +395,8 @@ __device__ __forceinline__ void mlp_raw32(const float xin[5], const Mlp32 &P, fl
         z1 = fmaf(h3[k], P.p[OFF_W4 + k * 3 + 1], z1);
         z2 = fmaf(h3[k], P.p[OFF_W4 + k * 3 + 2], z2);
     }
-    float b0 = 6.0f * tanhf(z0 / 6.0f), b1 = 6.0f * tanhf(z1 / 6.0f), b2 = 6.0f * tanhf(z2 / 6.0f);
+    const float sixth = 1.0f / 6.0f;
+    float b0 = 6.0f * tanh_fast(z0 * sixth), b1 = 6.0f * tanh_fast(z1 * sixth), b2 = 6.0f * tanh_fast(z2 * sixth);
     float bm = fmax_(fmax_(b0, b1), b2);
     float e0 = exp_fast(b0 - bm), e1 = exp_fast(b1 - bm), e2 = exp_fast(b2 - bm);
     float inv = rcp_fast(e0 + e1 + e2);
