@@ -118,10 +118,13 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
                                  double *dts, int32_t n_steps, void *workspace, size_t workspace_bytes,
                                  void *stream, double *sweep_ms, int32_t *sweep_launches);
 
-/* Closure counters accumulated since loop_init when WSR_FLAG_COUNT_CLOSURE is set:
- * out4 = {normal reconstructions, of which closure evaluated, Gauss-pair
- * reconstructions, of which closure evaluated} (synchronises the stream). */
-int wsr_euler2d_counters(void *workspace, uint64_t *out4, void *stream);
+/* Closure counters accumulated since loop_init when WSR_FLAG_COUNT_CLOSURE is set;
+ * copies the first n (1..6) of {normal reconstructions, of which closure
+ * evaluated, Gauss-pair reconstructions, of which closure evaluated, lanes of
+ * the waves that ran the normal closure, lanes of the waves that ran the Gauss
+ * closure} to out (synchronises the stream).  No counterpart in the reference:
+ * an instrument for the closure fraction that the roofline figures use. */
+int wsr_euler2d_counters(void *workspace, uint64_t *out, int32_t n, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* 1-D periodic scalar advection (the training / GSTE path)                  */

@@ -162,6 +162,31 @@ def test_loop_stops_on_nan(gpu_device):
 
 
 @pytest.mark.parametrize("scheme", ["js", "sr32"])
+def test_closure_counters(mlp32, gpu_device, scheme):
+    """Counting does not change the result; the counts are consistent."""
+    nx, ny = 48, 40
+    p = Params(nx=nx, ny=ny, x_length=1.0, y_length=1.0, cfl=0.4, t_end=0.02)
+    u0 = make_quadrant_initial(nx, ny, "c4")
+    mlp = mlp_for(scheme, None, mlp32)
+    plain = solver(p, scheme, mlp)
+    ua = plain.new_state(u0)
+    plain.run(ua, p.t_end)
+    counting = solver(p, scheme, mlp)
+    counting.set_counting(True)
+    ub = counting.new_state(u0)
+    counting.run(ub, p.t_end)
+    assert torch.equal(ua, ub)
+    c = counting.counters()
+    print(f"counters {scheme}: {c}")
+    assert c[0] > 0 and c[2] > 0
+    assert c[4] % 64 == 0 and c[5] % 64 == 0
+    if scheme == "js":
+        assert c[1] == c[3] == c[4] == c[5] == 0
+    else:
+        assert 0 < c[1] <= c[4] <= c[0] and 0 < c[3] <= c[5]
+
+
+@pytest.mark.parametrize("scheme", ["js", "sr32"])
 def test_full_size_one_step_parity(oracle, mlp32, gpu_device, scheme):
     """400x400 C.4 (the benchmark workload): one SSPRK3 step vs the oracle."""
     nx = ny = 400

@@ -84,7 +84,7 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler2d_loop_steps_timed.argtypes = [CFG, P, P, P, P, I32, P, SZ, P, ctypes.POINTER(D),
                                                    ctypes.POINTER(I32)]
         L.wsr_euler2d_counters.restype = ctypes.c_int
-        L.wsr_euler2d_counters.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), P]
+        L.wsr_euler2d_counters.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32, P]
         L.wsr_adv1d_rollout.restype = ctypes.c_int
         L.wsr_adv1d_rollout.argtypes = [ctypes.POINTER(Adv1DConfig), P, P, P, I32, P, P]
         A1 = ctypes.POINTER(Adv1DConfig)

@@ -37,11 +37,20 @@
 
 namespace wsr {
 
-constexpr int BLOCK = 256;
-constexpr int TN = 8;    // faces per tile along the sweep axis
-constexpr int TT = 28;   // transverse lines per tile (phase 1 covers TT + 4)
+constexpr int BLOCK = 256;   // element-wise kernels
+#ifndef WSR_SWEEP_TN
+#define WSR_SWEEP_TN 8
+#endif
+#ifndef WSR_SWEEP_TT
+#define WSR_SWEEP_TT 28
+#endif
+constexpr int TN = WSR_SWEEP_TN;   // faces per tile along the sweep axis
+constexpr int TT = WSR_SWEEP_TT;   // transverse lines per tile (phase 1 covers TT + 4)
 constexpr int NT1 = TT + 4;
-constexpr int NN = TN + 5;  // stencil cells per transverse line in a tile
+constexpr int NN = TN + 5;         // stencil cells per transverse line in a tile
+constexpr int SBLOCK = NT1 * TN;   // sweep threads: one face-line of phase 1 each
+static_assert(SBLOCK % 64 == 0 && SBLOCK <= 1024, "sweep block must be whole waves");
+constexpr int NCOUNT = 6;
 
 struct Ctl {
     double t, dt, t_end, cfl_h, dt_last;
@@ -49,8 +58,9 @@ struct Ctl {
     int32_t step, done, nan_flag, max_steps;
     // closure evaluations (WSR_FLAG_COUNT_CLOSURE): [0] normal reconstructions,
     // [1] of which evaluated the closure (not a plateau), [2] Gauss-pair
-    // reconstructions, [3] of which evaluated the closure
-    unsigned long long counters[4];
+    // reconstructions, [3] of which evaluated the closure, [4] / [5] lanes of
+    // the waves that ran the closure code (normal / Gauss; 64 per wave pass)
+    unsigned long long counters[NCOUNT];
 };
 
 struct Grid {
@@ -132,10 +142,23 @@ struct SweepArgs {
 };
 
 // wave-level count of lanes with `flag` set, added to *dst by one lane
-__device__ __forceinline__ void count_lanes(unsigned long long *dst, bool flag) {
-    unsigned long long m = __ballot(flag);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(dst, (unsigned long long)__popcll(m));
-}
+// per-lane closure counts, summed over the wave and added once per wave at the end
+struct LaneCounts {
+    unsigned c[NCOUNT] = {0u, 0u, 0u, 0u, 0u, 0u};
+    __device__ __forceinline__ void add(int slot, int wave_slot, bool counted, bool evaluated) {
+        c[slot] += counted ? 1u : 0u;
+        c[slot + 1] += (counted && evaluated) ? 1u : 0u;
+        c[wave_slot] += __ballot(evaluated) != 0ull ? 1u : 0u;
+    }
+    __device__ __forceinline__ void flush(unsigned long long *dst) {
+#pragma unroll
+        for (int i = 0; i < NCOUNT; ++i) {
+            unsigned v = c[i];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+            if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst + i, (unsigned long long)v);
+        }
+    }
+};
 
 template <int DIR>
 __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
@@ -147,17 +170,17 @@ __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
 //   FV  [NT1][TN][8]        face values, phase 1 -> phase 2
 //   REG phase 1: T [NN][NT1][4] stencil cells + EIG [N1][4] Roe eigenvalues
 //       phase 2: GV [CH][2][8] Gauss-point values + FL [CH][2][4] fluxes (one chunk)
-constexpr int N1 = NT1 * TN;     // face-lines of phase 1 (256: one per thread)
+constexpr int N1 = NT1 * TN;     // face-lines of phase 1 (one per thread)
 constexpr int N2 = TT * TN;      // flux faces of phase 2
-constexpr int CH = BLOCK / 2;    // phase-2 chunk: one (face, Gauss point) flux per thread
+constexpr int CH = SBLOCK / 2;   // phase-2 chunk: one (face, Gauss point) flux per thread
 constexpr int FV_SIZE = NT1 * TN * 8;
 constexpr int P1_SIZE = NN * NT1 * 4 + N1 * 4;
 constexpr int P2_SIZE = CH * 16 + CH * 8;
 constexpr int REG_SIZE = P1_SIZE > P2_SIZE ? P1_SIZE : P2_SIZE;
-static_assert(N1 == BLOCK, "phase 1a assigns one face-line per thread");
+static_assert(N1 == SBLOCK, "phase 1a assigns one face-line per thread");
 
 template <int DIR, int SCHEME, int RIEMANN>
-__global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
+__global__ __launch_bounds__(SBLOCK) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
                                                  SweepArgs A, typename MlpFor<SCHEME>::type mlp) {
     double dt = A.dt_host;
     if (A.ctl) {
@@ -181,7 +204,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
     double *FL = GV + CH * 16;
 
     // cooperative tile load, contiguous along the padded row
-    for (int e = threadIdx.x; e < NN * NT1; e += BLOCK) {
+    for (int e = threadIdx.x; e < NN * NT1; e += SBLOCK) {
         int n, tau, prow, pcol;
         if (DIR == 1) { n = e % NN; tau = e / NN; prow = t0 + 1 + tau; pcol = f0 + n; }
         else { tau = e % NT1; n = e / NT1; prow = f0 + n; pcol = t0 + 1 + tau; }
@@ -211,7 +234,8 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
     const int slot = lane >> 3;      // 32 face items per pass
 
     // ---------------- phase 1b: characteristic projection, WENO face value, back-projection
-    for (int base = 0; base < N1; base += BLOCK / 8) {
+    LaneCounts cnt;
+    for (int base = 0; base < N1; base += SBLOCK / 8) {
         const int it = base + slot;
         int fl, tau;
         item_map<DIR>(it, NT1, fl, tau);
@@ -228,10 +252,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
         }
         bool evaluated;
         double val = recon_normal<SCHEME>(c, s == 0, mlp, A.eno, h_n, evaluated);
-        if (A.counters) {
-            count_lanes(A.counters + 0, true);
-            count_lanes(A.counters + 1, evaluated);
-        }
+        if (A.counters) cnt.add(0, 4, true, evaluated);
         const int qb = lane & ~3;
         double cf0 = __shfl(val, qb + 0), cf1 = __shfl(val, qb + 1);
         double cf2 = __shfl(val, qb + 2), cf3 = __shfl(val, qb + 3);
@@ -244,7 +265,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
     // ---------------- phase 2: transverse Gauss reconstruction, then the Riemann fluxes
     for (int c0 = 0; c0 < N2; c0 += CH) {
         // 2a: both Gauss-point values of each (face, side, component), 8 lanes per face
-        for (int base = 0; base < CH; base += BLOCK / 8) {
+        for (int base = 0; base < CH; base += SBLOCK / 8) {
             const int ci = base + slot;
             const int it = c0 + ci;
             const bool valid = it < N2;
@@ -256,11 +277,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
             double g1, g2;
             bool evaluated;
             recon_gauss<SCHEME>(q, mlp, A.eno, h_t, g1, g2, evaluated);
-            if (A.counters) {
-                const bool owned = valid && (f0 + fl) < nfaces && (t0 + tl) < ntrans;
-                count_lanes(A.counters + 2, owned);
-                count_lanes(A.counters + 3, owned && evaluated);
-            }
+            if (A.counters) cnt.add(2, 5, valid && (f0 + fl) < nfaces && (t0 + tl) < ntrans, evaluated);
             GV[ci * 16 + gi] = g1;
             GV[ci * 16 + 8 + gi] = g2;
         }
@@ -279,7 +296,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
         }
         __syncthreads();
         // 2c: (F(g1) + F(g2)) dt/h, coalesced along the padded row
-        for (int e = threadIdx.x; e < CH * 4; e += BLOCK) {
+        for (int e = threadIdx.x; e < CH * 4; e += SBLOCK) {
             const int ci = e >> 2, kk = e & 3;
             const int it = c0 + ci;
             if (it >= N2) continue;
@@ -294,6 +311,7 @@ __global__ __launch_bounds__(BLOCK) void k_sweep(const double *__restrict__ bc, 
         }
         __syncthreads();
     }
+    if (A.counters) cnt.flush(A.counters);
 }
 
 // ------------------------------------------------------------------ update
@@ -391,7 +409,7 @@ __global__ void k_reset_speed(Ctl *ctl) {
 __global__ void k_loop_init(Ctl *ctl, double t_end, double cfl_h, int max_steps) {
     ctl->t = 0.0; ctl->dt = 0.0; ctl->t_end = t_end; ctl->cfl_h = cfl_h; ctl->dt_last = 0.0;
     ctl->speed_bits = 0ull; ctl->step = 0; ctl->done = 0; ctl->nan_flag = 0; ctl->max_steps = max_steps;
-    for (int i = 0; i < 4; ++i) ctl->counters[i] = 0ull;
+    for (int i = 0; i < NCOUNT; ++i) ctl->counters[i] = 0ull;
 }
 
 // dt of the next step (warp_weno5_helpers.py:189-207 and the loop head of run_to_time)
@@ -485,7 +503,7 @@ static void launch_sweep_t(const double *bc, double *F, const SweepArgs &A, cons
     const int nfaces = (DIR == 1 ? A.g.nx : A.g.ny) + 1;
     const int ntrans = DIR == 1 ? A.g.ny : A.g.nx;
     dim3 grid((nfaces + TN - 1) / TN, (ntrans + TT - 1) / TT);
-    hipLaunchKernelGGL((k_sweep<DIR, SCHEME, RIEMANN>), grid, dim3(BLOCK), 0, st, bc, F, A, mlp);
+    hipLaunchKernelGGL((k_sweep<DIR, SCHEME, RIEMANN>), grid, dim3(SBLOCK), 0, st, bc, F, A, mlp);
 }
 
 // optional per-launch timing of the sweep kernels (HIP events on the launch stream)
@@ -658,14 +676,14 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
     return rc;
 }
 
-int wsr_euler2d_counters(void *workspace, uint64_t *out4, void *stream) {
-    if (!workspace || !out4) return WSR_E_ARG;
+int wsr_euler2d_counters(void *workspace, uint64_t *out, int32_t n, void *stream) {
+    if (!workspace || !out || n < 1 || n > NCOUNT) return WSR_E_ARG;
     Ctl *ctl = (Ctl *)workspace;
     hipStream_t st = (hipStream_t)stream;
     Ctl host;
     if (hipMemcpyAsync(&host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st) != hipSuccess) return WSR_E_HIP;
     if (hipStreamSynchronize(st) != hipSuccess) return WSR_E_HIP;
-    for (int i = 0; i < 4; ++i) out4[i] = host.counters[i];
+    for (int i = 0; i < n; ++i) out[i] = host.counters[i];
     return WSR_OK;
 }
 

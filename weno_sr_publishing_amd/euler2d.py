@@ -159,8 +159,10 @@ class Euler2DSolver:
         self.cfg.flags = (self.cfg.flags | _lib.FLAG_COUNT_CLOSURE) if on else (self.cfg.flags & ~_lib.FLAG_COUNT_CLOSURE)
 
     def counters(self) -> list[int]:
-        out = (ctypes.c_uint64 * 4)()
-        rc = self._lib.wsr_euler2d_counters(self.workspace.data_ptr(), out, self._stream())
+        """[normal recons, normal closures, Gauss recons, Gauss closures,
+        lanes of waves running the normal closure, same for Gauss]."""
+        out = (ctypes.c_uint64 * 6)()
+        rc = self._lib.wsr_euler2d_counters(self.workspace.data_ptr(), out, 6, self._stream())
         _lib.check(rc, "wsr_euler2d_counters")
         return [int(v) for v in out]
 
