@@ -158,6 +158,7 @@ __device__ __forceinline__ void max_bwd(double a, double b, double g, double &ga
 
 // reverse pass of features_1d (non-plateau stencil): fbar (5) -> qbar (5, accumulated)
 __device__ void features_1d_backward(const double q[5], const double fbar[5], double qbar[5]) {
+#pragma clang fp contract(off)
     const double c1312 = 13.0 / 12.0, eps = 1.0e-15;
     double d2[3] = {q[0] - 2.0 * q[1] + q[2], q[1] - 2.0 * q[2] + q[3], q[2] - 2.0 * q[3] + q[4]};
     double e[3] = {q[0] - 4.0 * q[1] + 3.0 * q[2], q[1] - q[3], 3.0 * q[2] - 4.0 * q[3] + q[4]};

@@ -21,7 +21,12 @@ static inline float host_bf16(float x) {
 }
 
 // weno5_core.py:133-137 plateau_mask
+// The feature arithmetic is evaluated unfused, operation by operation as torch
+// does, so that exact ties (a max of equal second differences, a zero
+// difference under abs) fall the same way and the adjoint picks the same
+// subgradient as autograd.
 __device__ __forceinline__ bool plateau5(const double q[5]) {
+#pragma clang fp contract(off)
     double d20 = q[0] - 2.0 * q[1] + q[2];
     double d21 = q[1] - 2.0 * q[2] + q[3];
     double d22 = q[2] - 2.0 * q[3] + q[4];
@@ -36,6 +41,7 @@ __device__ __forceinline__ bool plateau5(const double q[5]) {
 
 // plateau test and features of one stencil; false = plateau (features not formed)
 __device__ __forceinline__ bool features_1d(const double q[5], double f[5]) {
+#pragma clang fp contract(off)
     double d20 = q[0] - 2.0 * q[1] + q[2];
     double d21 = q[1] - 2.0 * q[2] + q[3];
     double d22 = q[2] - 2.0 * q[3] + q[4];

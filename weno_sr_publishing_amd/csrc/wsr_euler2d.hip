@@ -39,7 +39,7 @@ namespace wsr {
 
 constexpr int BLOCK = 256;   // element-wise kernels
 #ifndef WSR_SWEEP_TN
-#define WSR_SWEEP_TN 8
+#define WSR_SWEEP_TN 4
 #endif
 #ifndef WSR_SWEEP_TT
 #define WSR_SWEEP_TT 28
@@ -181,7 +181,7 @@ static_assert(N1 == SBLOCK, "phase 1a assigns one face-line per thread");
 
 template <int DIR, int SCHEME, int RIEMANN>
 __global__ __launch_bounds__(SBLOCK) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
-                                                 SweepArgs A, typename MlpFor<SCHEME>::type mlp) {
+                                                 SweepArgs A, typename MlpSweep<SCHEME>::type mlp) {
     double dt = A.dt_host;
     if (A.ctl) {
         if (A.ctl->done) return;
@@ -435,6 +435,7 @@ __global__ void k_dt(Ctl *ctl) {
 // ----------------------------------------------------------- host side
 struct Workspace {
     Ctl *ctl;
+    void *mlp;          // closure parameters, neuron-major (FP64 or FP32 / BF16-rounded FP32)
     double *bc, *Fx, *Fy;
 };
 
@@ -448,6 +449,8 @@ static size_t ws_layout(int nx, int ny, Workspace *w, void *base) {
     char *b = (char *)base;
     if (w) w->ctl = (Ctl *)(b + off);
     off += align256(sizeof(Ctl));
+    if (w) w->mlp = (void *)(b + off);
+    off += align256(MLP5_COUNT * sizeof(double));
     if (w) w->bc = (double *)(b + off);
     off += align256(nstate);
     if (w) w->Fx = (double *)(b + off);
@@ -486,20 +489,44 @@ static Grid make_grid(const wsr_euler2d_config *cfg) {
     return g;
 }
 
-template <int DIR, int SCHEME, int RIEMANN>
-static void launch_sweep_t(const double *bc, double *F, const SweepArgs &A, const void *mlp_host,
-                           hipStream_t st) {
-    using MLP = typename MlpFor<SCHEME>::type;
-    MLP mlp;
-    memset(&mlp, 0, sizeof mlp);
-    if constexpr (SCHEME == SCHEME_SR64) {
-        memcpy(mlp.p, mlp_host, sizeof mlp.p);
-    } else if constexpr (SCHEME == SCHEME_SR32) {
-        memcpy(mlp.p, mlp_host, sizeof mlp.p);
-    } else if constexpr (SCHEME == SCHEME_SRBF16) {
-        const float *src = (const float *)mlp_host;
-        for (int i = 0; i < MLP5_COUNT; ++i) mlp.p[i] = host_round_bf16(src[i]);
+struct ParamWords { uint32_t w[MLP5_COUNT * 2]; };
+
+__global__ void k_put_params(ParamWords src, uint32_t *dst, int nwords) {
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) dst[i] = src.w[i];
+}
+
+// The closure parameters travel by value in the launch and land in the
+// workspace, neuron-major, ahead of the sweeps that read them (stream order).
+static void enqueue_params(const wsr_euler2d_config *cfg, const Workspace &w, hipStream_t st) {
+    const int scheme = cfg->scheme;
+    if (scheme != SCHEME_SR64 && scheme != SCHEME_SR32 && scheme != SCHEME_SRBF16) return;
+    ParamWords pw;
+    memset(&pw, 0, sizeof pw);
+    int nwords;
+    if (scheme == SCHEME_SR64) {
+        double dev[MLP5_COUNT];
+        pack_neuron_major((const double *)cfg->mlp, dev);
+        memcpy(pw.w, dev, sizeof dev);
+        nwords = MLP5_COUNT * 2;
+    } else {
+        float ref[MLP5_COUNT], dev[MLP5_COUNT];
+        memcpy(ref, cfg->mlp, sizeof ref);
+        if (scheme == SCHEME_SRBF16)
+            for (int i = 0; i < MLP5_COUNT; ++i) ref[i] = host_round_bf16(ref[i]);
+        pack_neuron_major(ref, dev);
+        memcpy(pw.w, dev, sizeof dev);
+        nwords = MLP5_COUNT;
     }
+    hipLaunchKernelGGL(k_put_params, dim3(1), dim3(256), 0, st, pw, (uint32_t *)w.mlp, nwords);
+}
+
+template <int DIR, int SCHEME, int RIEMANN>
+static void launch_sweep_t(const double *bc, double *F, const SweepArgs &A, const void *mlp_dev,
+                           hipStream_t st) {
+    typename MlpSweep<SCHEME>::type mlp;
+    memset(&mlp, 0, sizeof mlp);
+    if constexpr (SCHEME == SCHEME_SR64) mlp.p = (const double *)mlp_dev;
+    else if constexpr (SCHEME == SCHEME_SR32 || SCHEME == SCHEME_SRBF16) mlp.p = (const float *)mlp_dev;
     const int nfaces = (DIR == 1 ? A.g.nx : A.g.ny) + 1;
     const int ntrans = DIR == 1 ? A.g.ny : A.g.nx;
     dim3 grid((nfaces + TN - 1) / TN, (ntrans + TT - 1) / TT);
@@ -514,12 +541,12 @@ struct SweepTimer {
 
 template <int DIR>
 static void launch_sweep(int scheme, int riemann, const double *bc, double *F, const SweepArgs &A,
-                         const void *mlp_host, hipStream_t st, SweepTimer *timer = nullptr) {
+                         const void *mlp_dev, hipStream_t st, SweepTimer *timer = nullptr) {
     if (timer && timer->used + 2 <= timer->cap) (void)hipEventRecord(timer->ev[timer->used], st);
 #define WSR_CASE(S)                                                                              \
     case S:                                                                                      \
-        if (riemann == RIEMANN_FORCE) launch_sweep_t<DIR, S, RIEMANN_FORCE>(bc, F, A, mlp_host, st); \
-        else launch_sweep_t<DIR, S, RIEMANN_HLLC>(bc, F, A, mlp_host, st);                       \
+        if (riemann == RIEMANN_FORCE) launch_sweep_t<DIR, S, RIEMANN_FORCE>(bc, F, A, mlp_dev, st); \
+        else launch_sweep_t<DIR, S, RIEMANN_HLLC>(bc, F, A, mlp_dev, st);                         \
         break;
     switch (scheme) {
         WSR_CASE(SCHEME_JS)
@@ -547,8 +574,8 @@ static void enqueue_stage(const wsr_euler2d_config *cfg, const Workspace &w, con
     SweepArgs A;
     A.g = g; A.dt_host = dt; A.ctl = ctl; A.eno = cfg->eno_cutoff;
     A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
-    launch_sweep<1>(cfg->scheme, cfg->riemann, w.bc, w.Fx, A, cfg->mlp, st, timer);
-    launch_sweep<2>(cfg->scheme, cfg->riemann, w.bc, w.Fy, A, cfg->mlp, st, timer);
+    launch_sweep<1>(cfg->scheme, cfg->riemann, w.bc, w.Fx, A, w.mlp, st, timer);
+    launch_sweep<2>(cfg->scheme, cfg->riemann, w.bc, w.Fy, A, w.mlp, st, timer);
     const int ncells = g.nx * g.ny;
     hipLaunchKernelGGL(k_update, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const double *)w.bc, u0,
                        u_out, (const double *)w.Fx, (const double *)w.Fy, g, rk, ctl, fold_speed, dts);
@@ -575,6 +602,7 @@ int wsr_euler2d_stage(const wsr_euler2d_config *cfg, const double *u_stage, cons
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
     hipStream_t st = (hipStream_t)stream;
+    enqueue_params(cfg, w, st);
     enqueue_stage(cfg, w, u_stage, u0, u_out, dt, rk, boundary_time, nullptr, 0, nullptr, st);
     return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
 }
@@ -626,6 +654,7 @@ int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1,
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
     hipStream_t st = (hipStream_t)stream;
+    if (n_steps > 0) enqueue_params(cfg, w, st);
     for (int n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
         enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st);
@@ -650,6 +679,7 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
     timer.cap = 6 * 2 * n_steps;
     for (int i = 0; i < timer.cap; ++i)
         if (hipEventCreate(&timer.ev[i]) != hipSuccess) return WSR_E_HIP;
+    if (n_steps > 0) enqueue_params(cfg, w, st);
     for (int n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
         enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st, &timer);

@@ -29,10 +29,46 @@ struct Mlp64 { double p[MLP5_COUNT]; };
 struct Mlp32 { float p[MLP5_COUNT]; };
 struct MlpNone { int unused; };
 
+// by-value parameter blocks (1-D kernels)
 template <int SCHEME> struct MlpFor { using type = MlpNone; };
 template <> struct MlpFor<SCHEME_SR64> { using type = Mlp64; };
 template <> struct MlpFor<SCHEME_SR32> { using type = Mlp32; };
 template <> struct MlpFor<SCHEME_SRBF16> { using type = Mlp32; };
+
+// Device-resident parameter block of the 2-D sweeps, neuron-major: for every
+// layer and output neuron o the run [b_o, w_0o, ..., w_(K-1)o], so that one
+// neuron's weights are contiguous and arrive through scalar loads.
+constexpr int DEV_L1 = 0, DEV_L2 = 60, DEV_L3 = 126, DEV_L4 = 168;
+template <class T> struct MlpK { const T *p; };
+template <int SCHEME> struct MlpSweep { using type = MlpNone; };
+template <> struct MlpSweep<SCHEME_SR64> { using type = MlpK<double>; };
+template <> struct MlpSweep<SCHEME_SR32> { using type = MlpK<float>; };
+template <> struct MlpSweep<SCHEME_SRBF16> { using type = MlpK<float>; };
+
+// reference layout -> neuron-major (host or device)
+template <class T>
+__host__ __device__ inline void pack_neuron_major(const T *ref, T *dev) {
+    const int K[4] = {5, 10, 6, 6}, N[4] = {10, 6, 6, 3};
+    const int W[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, B[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
+    int o = 0;
+    for (int l = 0; l < 4; ++l)
+        for (int n = 0; n < N[l]; ++n) {
+            dev[o++] = ref[B[l] + n];
+            for (int k = 0; k < K[l]; ++k) dev[o++] = ref[W[l] + k * N[l] + n];
+        }
+}
+
+// Parameters are read through the constant address space (scalar loads into
+// SGPRs, one SGPR operand per FMA).  The empty asm makes the pointer opaque
+// before every layer so that the loads stay next to their use instead of being
+// hoisted to the kernel entry (which would exceed the SGPR file and spill).
+template <class T> using kconst_ptr = const T __attribute__((address_space(4))) *;
+template <class T>
+__device__ __forceinline__ kconst_ptr<T> kparams(const T *p) {
+    kconst_ptr<T> k = (kconst_ptr<T>)p;
+    asm volatile("" : "+s"(k));
+    return k;
+}
 
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
@@ -307,41 +343,45 @@ __device__ __forceinline__ bool sr_features(const double q[5], double f[5]) {
 // activations, r = softmax(6 tanh(z / 6)).
 __device__ __forceinline__ double swish64(double x) { return x / (1.0 + exp(-x)); }
 
-__device__ __forceinline__ void mlp_raw64(const double x[5], const Mlp64 &P, double r[3]) {
-    double h1[10], h2[6], h3[6];
+// Two raw evaluations (direct and reflected input) side by side: every weight
+// read feeds two independent FMA chains.
+template <int K, int N, class T, class ACT>
+__device__ __forceinline__ void dense_pair(const T *xa, const T *xb, const T *w, int base, T *ha, T *hb, ACT act) {
+    kconst_ptr<T> c = kparams(w + base);
 #pragma unroll
-    for (int o = 0; o < 10; ++o) {
-        double z = P.p[OFF_B1 + o];
+    for (int o = 0; o < N; ++o) {
+        if constexpr (sizeof(T) == 8) c = kparams(w + base + o * (K + 1));   // FP64: one neuron at a time
+        const int b = sizeof(T) == 8 ? 0 : o * (K + 1);
+        T za = c[b], zb = c[b];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) z = z + x[k] * P.p[OFF_W1 + k * 10 + o];
-        h1[o] = swish64(z);
+        for (int k = 0; k < K; ++k) {
+            za = za + xa[k] * c[b + 1 + k];
+            zb = zb + xb[k] * c[b + 1 + k];
+        }
+        ha[o] = act(za);
+        hb[o] = act(zb);
     }
-#pragma unroll
-    for (int o = 0; o < 6; ++o) {
-        double z = P.p[OFF_B2 + o];
-#pragma unroll
-        for (int k = 0; k < 10; ++k) z = z + h1[k] * P.p[OFF_W2 + k * 6 + o];
-        h2[o] = swish64(z);
-    }
-#pragma unroll
-    for (int o = 0; o < 6; ++o) {
-        double z = P.p[OFF_B3 + o];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) z = z + h2[k] * P.p[OFF_W3 + k * 6 + o];
-        h3[o] = swish64(z);
-    }
-    double z0 = P.p[OFF_B4 + 0], z1 = P.p[OFF_B4 + 1], z2 = P.p[OFF_B4 + 2];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        z0 = z0 + h3[k] * P.p[OFF_W4 + k * 3 + 0];
-        z1 = z1 + h3[k] * P.p[OFF_W4 + k * 3 + 1];
-        z2 = z2 + h3[k] * P.p[OFF_W4 + k * 3 + 2];
-    }
-    double b0 = 6.0 * tanh(z0 / 6.0), b1 = 6.0 * tanh(z1 / 6.0), b2 = 6.0 * tanh(z2 / 6.0);
+}
+
+__device__ __forceinline__ void softmax_head64(const double z[3], double r[3]) {
+    double b0 = 6.0 * tanh(z[0] / 6.0), b1 = 6.0 * tanh(z[1] / 6.0), b2 = 6.0 * tanh(z[2] / 6.0);
     double bm = dmax(dmax(b0, b1), b2);
     double e0 = exp(b0 - bm), e1 = exp(b1 - bm), e2 = exp(b2 - bm);
     double inv = 1.0 / (e0 + e1 + e2);
     r[0] = e0 * inv; r[1] = e1 * inv; r[2] = e2 * inv;
+}
+
+__device__ __forceinline__ void mlp_pair64(const double xa[5], const double xb[5], const double *w, double ra[3],
+                                           double rb[3]) {
+    double a1[10], b1[10], a2[6], b2[6], a3[6], b3[6], za[3], zb[3];
+    auto sw = [](double z) { return swish64(z); };
+    auto id = [](double z) { return z; };
+    dense_pair<5, 10>(xa, xb, w, DEV_L1, a1, b1, sw);
+    dense_pair<10, 6>(a1, b1, w, DEV_L2, a2, b2, sw);
+    dense_pair<6, 6>(a2, b2, w, DEV_L3, a3, b3, sw);
+    dense_pair<6, 3>(a3, b3, w, DEV_L4, za, zb, id);
+    softmax_head64(za, ra);
+    softmax_head64(zb, rb);
 }
 
 // FP32 closure on the VALU: v_exp_f32 / v_rcp_f32 for the swish, the bounded
@@ -362,45 +402,33 @@ __device__ __forceinline__ float round_bf16(float x) {
     return __uint_as_float(u);
 }
 
-template <bool BF16>
-__device__ __forceinline__ void mlp_raw32(const float xin[5], const Mlp32 &P, float r[3]) {
-    float x[5], h1[10], h2[6], h3[6];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) x[k] = BF16 ? round_bf16(xin[k]) : xin[k];
-#pragma unroll
-    for (int o = 0; o < 10; ++o) {
-        float z = P.p[OFF_B1 + o];
-#pragma unroll
-        for (int k = 0; k < 5; ++k) z = fmaf(x[k], P.p[OFF_W1 + k * 10 + o], z);
-        h1[o] = BF16 ? round_bf16(swish32(z)) : swish32(z);
-    }
-#pragma unroll
-    for (int o = 0; o < 6; ++o) {
-        float z = P.p[OFF_B2 + o];
-#pragma unroll
-        for (int k = 0; k < 10; ++k) z = fmaf(h1[k], P.p[OFF_W2 + k * 6 + o], z);
-        h2[o] = BF16 ? round_bf16(swish32(z)) : swish32(z);
-    }
-#pragma unroll
-    for (int o = 0; o < 6; ++o) {
-        float z = P.p[OFF_B3 + o];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) z = fmaf(h2[k], P.p[OFF_W3 + k * 6 + o], z);
-        h3[o] = BF16 ? round_bf16(swish32(z)) : swish32(z);
-    }
-    float z0 = P.p[OFF_B4 + 0], z1 = P.p[OFF_B4 + 1], z2 = P.p[OFF_B4 + 2];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        z0 = fmaf(h3[k], P.p[OFF_W4 + k * 3 + 0], z0);
-        z1 = fmaf(h3[k], P.p[OFF_W4 + k * 3 + 1], z1);
-        z2 = fmaf(h3[k], P.p[OFF_W4 + k * 3 + 2], z2);
-    }
+__device__ __forceinline__ void softmax_head32(const float z[3], float r[3]) {
     const float sixth = 1.0f / 6.0f;
-    float b0 = 6.0f * tanh_fast(z0 * sixth), b1 = 6.0f * tanh_fast(z1 * sixth), b2 = 6.0f * tanh_fast(z2 * sixth);
+    float b0 = 6.0f * tanh_fast(z[0] * sixth), b1 = 6.0f * tanh_fast(z[1] * sixth), b2 = 6.0f * tanh_fast(z[2] * sixth);
     float bm = fmax_(fmax_(b0, b1), b2);
     float e0 = exp_fast(b0 - bm), e1 = exp_fast(b1 - bm), e2 = exp_fast(b2 - bm);
     float inv = rcp_fast(e0 + e1 + e2);
     r[0] = e0 * inv; r[1] = e1 * inv; r[2] = e2 * inv;
+}
+
+// FP32 (BF16: inputs, weights and activations rounded to bfloat16, FP32 arithmetic)
+template <bool BF16>
+__device__ __forceinline__ void mlp_pair32(const float xa_in[5], const float xb_in[5], const float *w, float ra[3],
+                                           float rb[3]) {
+    float xa[5], xb[5], a1[10], b1[10], a2[6], b2[6], a3[6], b3[6], za[3], zb[3];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        xa[k] = BF16 ? round_bf16(xa_in[k]) : xa_in[k];
+        xb[k] = BF16 ? round_bf16(xb_in[k]) : xb_in[k];
+    }
+    auto sw = [](float z) { return BF16 ? round_bf16(swish32(z)) : swish32(z); };
+    auto id = [](float z) { return z; };
+    dense_pair<5, 10>(xa, xb, w, DEV_L1, a1, b1, sw);
+    dense_pair<10, 6>(a1, b1, w, DEV_L2, a2, b2, sw);
+    dense_pair<6, 6>(a2, b2, w, DEV_L3, a3, b3, sw);
+    dense_pair<6, 3>(a3, b3, w, DEV_L4, za, zb, id);
+    softmax_head32(za, ra);
+    softmax_head32(zb, rb);
 }
 
 // Reflection-averaged ratios 0.5 [M(x) + P3 M(P5 x)] in FP64 (C64:225-237, C32:225-237).
@@ -409,14 +437,12 @@ __device__ __forceinline__ void sr_ratios(const double f[5], const MLP &P, doubl
     double direct[3], refl[3];
     if constexpr (SCHEME == SCHEME_SR64) {
         double xr[5] = {f[2], f[1], f[0], f[3], f[4]};
-        mlp_raw64(f, P, direct);
-        mlp_raw64(xr, P, refl);
+        mlp_pair64(f, xr, P.p, direct, refl);
     } else {
         float xd[5] = {(float)f[0], (float)f[1], (float)f[2], (float)f[3], (float)f[4]};
         float xr[5] = {xd[2], xd[1], xd[0], xd[3], xd[4]};
         float rd[3], rr[3];
-        mlp_raw32<SCHEME == SCHEME_SRBF16>(xd, P, rd);
-        mlp_raw32<SCHEME == SCHEME_SRBF16>(xr, P, rr);
+        mlp_pair32<SCHEME == SCHEME_SRBF16>(xd, xr, P.p, rd, rr);
         direct[0] = rd[0]; direct[1] = rd[1]; direct[2] = rd[2];
         refl[0] = rr[0]; refl[1] = rr[1]; refl[2] = rr[2];
     }
@@ -458,7 +484,7 @@ template <int SCHEME, class MLP>
 __device__ __forceinline__ double recon_normal(const double q[5], bool lr1, const MLP &P, int eno, double h,
                                                bool &closure_evaluated) {
     double s[3], d[3];
-    closure_evaluated = true;
+    closure_evaluated = false;
     candidates_normal(q, lr1, s);
     d_normal(lr1, d);
     if constexpr (SCHEME == SCHEME_JS) {
@@ -491,7 +517,7 @@ template <int SCHEME, class MLP>
 __device__ __forceinline__ void recon_gauss(const double q[5], const MLP &P, int eno, double h,
                                             double &g1, double &g2, bool &closure_evaluated) {
     double s1[3], s2[3], d1[3], d2[3];
-    closure_evaluated = true;
+    closure_evaluated = false;
     candidates_gauss(q, s1, s2);
     d_gauss(d1, d2);
     if constexpr (SCHEME == SCHEME_JS) {
