@@ -73,6 +73,9 @@ def parse():
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="after the run, copy one MIB-sized FP64 buffer into another (a reference kernel "
+                         "of known traffic for hardware-counter byte calibration)")
     return ap.parse_args()
 
 
@@ -265,6 +268,13 @@ def main() -> int:
                                "cpu_model": cpu_model, "host_cpus": os.cpu_count()}
         out["gpu_vs_cpu"] = value / cpu_rate
 
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": n * 8, "write": n * 8}
     print(json.dumps(out), flush=True)
     return 0
 
