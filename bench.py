@@ -52,6 +52,10 @@ GAUSS = {"js": 101, "z": 106,
 FACE_LINE_FIXED = 68 + 42 + 40 * 8 + 8 * 7   # Roe + eigenvalues, left rows, 40 projections, back-projection
 FLUX_FACE_FIXED = 2 * 110 + 12               # two HLLC fluxes + dt/h scaling
 BYTES_PER_SWEEP_CELL = 32 + 32               # state tile read + flux write (FP64 x 4)
+# HBM bytes per SR32 sweep launch at 400^2 (mean of x and y), from the two
+# rocprofv3 counter passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000 after the
+# copy-kernel calibration) recorded in profiles/pmc_traffic_k_sweep_c4_400x400.md
+SWEEP_TRAFFIC_BYTES_400 = 21.99e6
 
 
 def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -> float:
@@ -202,7 +206,8 @@ def main() -> int:
             "peak": PEAK_VALU_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_VALU_TFLOPS,
-            "traffic": None,
+            "traffic": SWEEP_TRAFFIC_BYTES_400 if args.closure == "sr32" else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": avg_ms,
             "launches": sweep_n,
             "algorithmic_flops_per_launch": flops,

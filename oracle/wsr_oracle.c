@@ -1175,3 +1175,675 @@ int oracle_adv1d_run(double *u, int batch, int n, const double *a, const double 
 double oracle_face_1d(const double *q, int lr, int scheme, const void *mlp) { return face_1d(q, lr, scheme, mlp); }
 
 void oracle_features_1d(const double *q, double *f) { features_1d(q, f); }
+
+/* ======================================================================== */
+/* WENO7 (order-7 WENO with a 6-24-16-16-4 closure) and Shu's downwind RK4   */
+/*   W7  = code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/weno7_core.py */
+/*   RK4 = code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/rk4_advection.py */
+/*   ZT  = code/snapshot/weno_z_borges_p2_results/torch_weno_z.py             */
+/*   WH7 = code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/ */
+/*         warp_weno7_ader4_helpers_classical_only.py                          */
+/*   EXT = .../warp_sod/warp_weno7_external_overwrite_fullall.py               */
+/*   BP  = .../warp_sod/beta_provider.py                                       */
+/*   P7  = code/snapshot/weno7_point_rk4_shu/point_rk4.py                      */
+/*   P7M = .../warp_sod/point_rk4_mlp.py                                       */
+/*   DM7 = code/snapshot/for_paper_results/solvers/weno7_double_mach.py        */
+/*   Z7  = code/snapshot/weno_z_borges_p2_results/weno7_helpers_z.py           */
+/* ======================================================================== */
+
+#define MLP7_COUNT 908
+/* parameter block: w1[6][24] b1[24] w2[24][16] b2[16] w3[16][16] b3[16] w4[16][4] b4[4] */
+enum { O7_W1 = 0, O7_B1 = 144, O7_W2 = 168, O7_B2 = 552, O7_W3 = 568, O7_B3 = 824, O7_W4 = 840, O7_B4 = 904 };
+enum { RIEMANN7_HLLC = 0, RIEMANN7_FORCE = 1, RIEMANN7_EVILIN = 2 };
+
+/* W7:28-103 candidate coefficients, heads 1..4 (1: x_{i+1/2}, 2: x_{i-1/2},
+ * 3: Gauss point -sqrt(3)/6, 4: +sqrt(3)/6) */
+static void w7_coeffs(int lr, double c[4][4]) {
+    const double s = sqrt(3.0);
+    if (lr == 1) {
+        double t[4][4] = {{-1.0 / 4.0, 13.0 / 12.0, -23.0 / 12.0, 25.0 / 12.0},
+                          {1.0 / 12.0, -5.0 / 12.0, 13.0 / 12.0, 1.0 / 4.0},
+                          {-1.0 / 12.0, 7.0 / 12.0, 7.0 / 12.0, -1.0 / 12.0},
+                          {1.0 / 4.0, 13.0 / 12.0, -5.0 / 12.0, 1.0 / 12.0}};
+        memcpy(c, t, sizeof t);
+    } else if (lr == 2) {
+        double t[4][4] = {{1.0 / 12.0, -5.0 / 12.0, 13.0 / 12.0, 1.0 / 4.0},
+                          {-1.0 / 12.0, 7.0 / 12.0, 7.0 / 12.0, -1.0 / 12.0},
+                          {1.0 / 4.0, 13.0 / 12.0, -5.0 / 12.0, 1.0 / 12.0},
+                          {25.0 / 12.0, -23.0 / 12.0, 13.0 / 12.0, -1.0 / 4.0}};
+        memcpy(c, t, sizeof t);
+    } else if (lr == 3) {
+        double t[4][4] = {{11.0 * s / 216.0, -17.0 * s / 72.0, 35.0 * s / 72.0, (216.0 - 65.0 * s) / 216.0},
+                          {-7.0 * s / 216.0, 13.0 * s / 72.0, (72.0 - 7.0 * s) / 72.0, -11.0 * s / 216.0},
+                          {11.0 * s / 216.0, (7.0 * s + 72.0) / 72.0, -13.0 * s / 72.0, 7.0 * s / 216.0},
+                          {(65.0 * s + 216.0) / 216.0, -35.0 * s / 72.0, 17.0 * s / 72.0, -11.0 * s / 216.0}};
+        memcpy(c, t, sizeof t);
+    } else {
+        double t[4][4] = {{-11.0 * s / 216.0, 17.0 * s / 72.0, -35.0 * s / 72.0, (65.0 * s + 216.0) / 216.0},
+                          {7.0 * s / 216.0, -13.0 * s / 72.0, (7.0 * s + 72.0) / 72.0, 11.0 * s / 216.0},
+                          {-11.0 * s / 216.0, (72.0 - 7.0 * s) / 72.0, 13.0 * s / 72.0, -7.0 * s / 216.0},
+                          {(216.0 - 65.0 * s) / 216.0, 35.0 * s / 72.0, -17.0 * s / 72.0, 11.0 * s / 216.0}};
+        memcpy(c, t, sizeof t);
+    }
+}
+
+/* W7:105-128 optimal weights */
+static void w7_optimal(int lr, double d[4]) {
+    const double s = sqrt(3.0);
+    if (lr == 1) { d[0] = 1.0 / 35.0; d[1] = 12.0 / 35.0; d[2] = 18.0 / 35.0; d[3] = 4.0 / 35.0; }
+    else if (lr == 2) { d[0] = 4.0 / 35.0; d[1] = 18.0 / 35.0; d[2] = 12.0 / 35.0; d[3] = 1.0 / 35.0; }
+    else if (lr == 3) {
+        d[0] = 59.0 / 880.0 + 5.0 * s / 16632.0; d[1] = 381.0 / 880.0 + 587.0 * s / 194040.0;
+        d[2] = 381.0 / 880.0 - 587.0 * s / 194040.0; d[3] = 59.0 / 880.0 - 5.0 * s / 16632.0;
+    } else {
+        d[0] = 59.0 / 880.0 - 5.0 * s / 16632.0; d[1] = 381.0 / 880.0 - 587.0 * s / 194040.0;
+        d[2] = 381.0 / 880.0 + 587.0 * s / 194040.0; d[3] = 59.0 / 880.0 + 5.0 * s / 16632.0;
+    }
+}
+
+/* W7:200-206 candidate_values: sum_j q[k+j] c[k][j] */
+static void w7_candidates(const double q[7], int lr, double s[4]) {
+    double c[4][4];
+    w7_coeffs(lr, c);
+    for (int k = 0; k < 4; ++k) {
+        double acc = q[k] * c[k][0];
+        for (int j = 1; j < 4; ++j) acc = acc + q[k + j] * c[k][j];
+        s[k] = acc;
+    }
+}
+
+/* W7:139-163 _delta_forms */
+static void w7_delta_forms(const double q[7], double f[4][3]) {
+    double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6];
+    f[0][0] = -2.0 * q0 + 9.0 * q1 - 18.0 * q2 + 11.0 * q3;
+    f[0][1] = -q0 + 4.0 * q1 - 5.0 * q2 + 2.0 * q3;
+    f[0][2] = -q0 + 3.0 * q1 - 3.0 * q2 + q3;
+    f[1][0] = q1 - 6.0 * q2 + 3.0 * q3 + 2.0 * q4;
+    f[1][1] = q2 - 2.0 * q3 + q4;
+    f[1][2] = -q1 + 3.0 * q2 - 3.0 * q3 + q4;
+    f[2][0] = -2.0 * q2 - 3.0 * q3 + 6.0 * q4 - q5;
+    f[2][1] = q2 - 2.0 * q3 + q4;
+    f[2][2] = -q2 + 3.0 * q3 - 3.0 * q4 + q5;
+    f[3][0] = -11.0 * q3 + 18.0 * q4 - 9.0 * q5 + 2.0 * q6;
+    f[3][1] = 2.0 * q3 - 5.0 * q4 + 4.0 * q5 - q6;
+    f[3][2] = -q3 + 3.0 * q4 - 3.0 * q5 + q6;
+}
+
+static const double W7_FORM_COEF[3] = {1.0 / 36.0, 13.0 / 12.0, 781.0 / 720.0};
+
+/* W7:180-187 delta_values: sum_j c_j |form_kj| */
+static void w7_delta_values(const double q[7], double delta[4]) {
+    double f[4][3];
+    w7_delta_forms(q, f);
+    for (int k = 0; k < 4; ++k)
+        delta[k] = fabs(f[k][0]) * W7_FORM_COEF[0] + fabs(f[k][1]) * W7_FORM_COEF[1] + fabs(f[k][2]) * W7_FORM_COEF[2];
+}
+
+/* W7:190-197 classical_beta: sum_j c_j form_kj^2 */
+static void w7_classical_beta(const double q[7], double beta[4]) {
+    double f[4][3];
+    w7_delta_forms(q, f);
+    for (int k = 0; k < 4; ++k)
+        beta[k] = f[k][0] * f[k][0] * W7_FORM_COEF[0] + f[k][1] * f[k][1] * W7_FORM_COEF[1] +
+                  f[k][2] * f[k][2] * W7_FORM_COEF[2];
+}
+
+static double w7_q_scale(const double q[7]) {
+    double m = fabs(q[0]);
+    for (int k = 1; k < 7; ++k) m = dmax(m, fabs(q[k]));
+    return dmax(m, 1.0);
+}
+
+/* W7:166-177 weno7_gamma_s */
+static double w7_gamma_s(const double q[7]) {
+    const double eps = 1.0e-15;
+    double m = 0.0;
+    for (int i = 0; i < 5; ++i) {
+        double d2 = q[i] - 2.0 * q[i + 1] + q[i + 2];
+        double den = fabs(q[i + 1] - q[i]) + fabs(q[i + 2] - q[i + 1]) + eps;
+        double g = fabs(d2) / den;
+        m = i == 0 ? g : dmax(m, g);
+    }
+    return dmin(dmax(m, 0.0), 1.0);
+}
+
+/* W7:199-215 weno7_features (no remap of the scale channel) */
+static void w7_features(const double q[7], double f[6]) {
+    double delta[4];
+    w7_delta_values(q, delta);
+    double dm = dmax(dmax(dmax(delta[0], delta[1]), delta[2]), delta[3]);
+    double den = dmax(dm, 1.0e-15);
+    for (int k = 0; k < 4; ++k) f[k] = delta[k] / den;
+    double rel = dmax(dm / w7_q_scale(q), 1.0e-30);
+    f[4] = w7_gamma_s(q);
+    f[5] = dmin(dmax((log10(rel) + 16.0) / 16.0, 0.0), 1.0);
+}
+
+/* W7:218-221 plateau_mask */
+static int w7_plateau(const double q[7]) {
+    double delta[4];
+    w7_delta_values(q, delta);
+    double dm = dmax(dmax(dmax(delta[0], delta[1]), delta[2]), delta[3]);
+    return dm <= 1.0e-13 * w7_q_scale(q);
+}
+
+/* W7:301-306 raw_forward: three swish layers, r = softmax(6 tanh(z / 6)) */
+static void w7_mlp_raw64(const double x[6], const double *p, double r[4]) {
+    double h1[24], h2[16], h3[16], raw[4];
+    for (int o = 0; o < 24; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 6; ++k) z = z + x[k] * p[O7_W1 + k * 24 + o];
+        z = z + p[O7_B1 + o];
+        h1[o] = z * (1.0 / (1.0 + exp(-z)));
+    }
+    for (int o = 0; o < 16; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 24; ++k) z = z + h1[k] * p[O7_W2 + k * 16 + o];
+        z = z + p[O7_B2 + o];
+        h2[o] = z * (1.0 / (1.0 + exp(-z)));
+    }
+    for (int o = 0; o < 16; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 16; ++k) z = z + h2[k] * p[O7_W3 + k * 16 + o];
+        z = z + p[O7_B3 + o];
+        h3[o] = z * (1.0 / (1.0 + exp(-z)));
+    }
+    for (int o = 0; o < 4; ++o) {
+        double z = 0.0;
+        for (int k = 0; k < 16; ++k) z = z + h3[k] * p[O7_W4 + k * 4 + o];
+        raw[o] = 6.0 * tanh((z + p[O7_B4 + o]) / 6.0);
+    }
+    double m = dmax(dmax(dmax(raw[0], raw[1]), raw[2]), raw[3]);
+    double e[4], s = 0.0;
+    for (int o = 0; o < 4; ++o) { e[o] = exp(raw[o] - m); s = s + e[o]; }
+    for (int o = 0; o < 4; ++o) r[o] = e[o] / s;
+}
+
+/* FP32 evaluation of the same network (features and weights cast to float) */
+static void w7_mlp_raw32(const double xd[6], const float *p, double r[4]) {
+    float x[6], h1[24], h2[16], h3[16], raw[4];
+    for (int k = 0; k < 6; ++k) x[k] = (float)xd[k];
+    for (int o = 0; o < 24; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 6; ++k) z = z + x[k] * p[O7_W1 + k * 24 + o];
+        z = z + p[O7_B1 + o];
+        h1[o] = z * (1.0f / (1.0f + expf(-z)));
+    }
+    for (int o = 0; o < 16; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 24; ++k) z = z + h1[k] * p[O7_W2 + k * 16 + o];
+        z = z + p[O7_B2 + o];
+        h2[o] = z * (1.0f / (1.0f + expf(-z)));
+    }
+    for (int o = 0; o < 16; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 16; ++k) z = z + h2[k] * p[O7_W3 + k * 16 + o];
+        z = z + p[O7_B3 + o];
+        h3[o] = z * (1.0f / (1.0f + expf(-z)));
+    }
+    for (int o = 0; o < 4; ++o) {
+        float z = 0.0f;
+        for (int k = 0; k < 16; ++k) z = z + h3[k] * p[O7_W4 + k * 4 + o];
+        raw[o] = 6.0f * tanhf((z + p[O7_B4 + o]) / 6.0f);
+    }
+    float m = fmaxf_(fmaxf_(fmaxf_(raw[0], raw[1]), raw[2]), raw[3]);
+    float e[4], s = 0.0f;
+    for (int o = 0; o < 4; ++o) { e[o] = expf(raw[o] - m); s = s + e[o]; }
+    for (int o = 0; o < 4; ++o) r[o] = (double)(e[o] / s);
+}
+
+static void w7_raw(const double f[6], int scheme, const void *mlp, double r[4]) {
+    if (scheme == SCHEME_SR64) w7_mlp_raw64(f, (const double *)mlp, r);
+    else w7_mlp_raw32(f, (const float *)mlp, r);
+}
+
+/* W7:308-311 forward: 0.5 [M(x) + P4 M(P6 x)], P6 reverses the four deltas */
+static void w7_sym_ratios(const double f[6], int scheme, const void *mlp, double r[4]) {
+    double fr[6] = {f[3], f[2], f[1], f[0], f[4], f[5]};
+    double d[4], e[4];
+    w7_raw(f, scheme, mlp, d);
+    w7_raw(fr, scheme, mlp, e);
+    for (int k = 0; k < 4; ++k) r[k] = 0.5 * (d[k] + e[3 - k]);
+}
+
+/* ---------------------------------------------------------------- 1-D path */
+/* W7:224-241 omega_from_ratio / classical_omega; ZT:24-33 omega7 (WENO7-Z) */
+static double w7_face_1d(const double q[7], int lr, int scheme, const void *mlp, double h) {
+    double s[4], d[4], w[4];
+    w7_candidates(q, lr, s);
+    w7_optimal(lr, d);
+    if (scheme == SCHEME_Z) {
+        double b[4];
+        w7_classical_beta(q, b);
+        double tau = fabs(-b[0] - 3.0 * b[1] + 3.0 * b[2] + b[3]);
+        double a[4], sum = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            double ratio = tau / (b[k] + h * h * h);
+            a[k] = d[k] * (1.0 + ratio * ratio * ratio);
+        }
+        sum = ((a[0] + a[1]) + a[2]) + a[3];
+        for (int k = 0; k < 4; ++k) w[k] = a[k] / sum;
+    } else if (scheme == SCHEME_JS) {
+        double b[4], a[4];
+        w7_classical_beta(q, b);
+        for (int k = 0; k < 4; ++k) a[k] = d[k] / ((b[k] + 1.0e-6) * (b[k] + 1.0e-6));
+        double sum = ((a[0] + a[1]) + a[2]) + a[3];
+        for (int k = 0; k < 4; ++k) w[k] = a[k] / sum;
+    } else if (w7_plateau(q)) {
+        for (int k = 0; k < 4; ++k) w[k] = d[k];
+    } else {
+        double f[6], r[4], a[4];
+        w7_features(q, f);
+        w7_sym_ratios(f, scheme, mlp, r);
+        for (int k = 0; k < 4; ++k) {
+            double beta = 4.0 * r[k];
+            a[k] = d[k] / ((beta + 1.0e-12) * (beta + 1.0e-12));
+        }
+        double sum = ((a[0] + a[1]) + a[2]) + a[3];
+        for (int k = 0; k < 4; ++k) w[k] = a[k] / sum;
+    }
+    return ((w[0] * s[0] + w[1] * s[1]) + w[2] * s[2]) + w[3] * s[3];
+}
+
+/* RK4:91-134 rhs_signed (reverse = the downwind operator L~) */
+static void w7_rhs_1d(const double *u, int n, double a, double dxinv, int scheme, const void *mlp, int reverse,
+                      double *face, double *rhs) {
+    const double h = 1.0 / dxinv;
+    for (int i = 0; i < n; ++i) {
+        double q[7];
+        int use_left = (a > 0.0) != (reverse != 0);   /* left state of face i+1/2: cell i, lr 1 */
+        if (use_left) {
+            for (int m = 0; m < 7; ++m) q[m] = u[((i - 3 + m) % n + n) % n];
+            face[i] = a * w7_face_1d(q, 1, scheme, mlp, h);
+        } else {
+            for (int m = 0; m < 7; ++m) q[m] = u[((i - 2 + m) % n + n) % n];
+            face[i] = a * w7_face_1d(q, 2, scheme, mlp, h);
+        }
+    }
+    for (int i = 0; i < n; ++i) rhs[i] = -(face[i] - face[(i - 1 + n) % n]) * dxinv;
+}
+
+/* RK4:137-195 shu_rk4_step_signed */
+static void w7_shu_rk4_1d(double *u, int n, double a, double dt, double dxinv, int scheme, const void *mlp,
+                          double *w) {
+    double *face = w, *l0 = w + n, *lt0 = w + 2 * n, *u1 = w + 3 * n, *l1 = w + 4 * n, *lt1 = w + 5 * n;
+    double *u2 = w + 6 * n, *l2 = w + 7 * n, *u3 = w + 8 * n, *l3 = w + 9 * n;
+    w7_rhs_1d(u, n, a, dxinv, scheme, mlp, 0, face, l0);
+    w7_rhs_1d(u, n, a, dxinv, scheme, mlp, 1, face, lt0);
+    for (int i = 0; i < n; ++i) u1[i] = u[i] + 0.5 * dt * l0[i];
+    w7_rhs_1d(u1, n, a, dxinv, scheme, mlp, 0, face, l1);
+    w7_rhs_1d(u1, n, a, dxinv, scheme, mlp, 1, face, lt1);
+    for (int i = 0; i < n; ++i)
+        u2[i] = (649.0 / 1600.0) * u[i] - (10890423.0 / 25193600.0) * dt * lt0[i] + (951.0 / 1600.0) * u1[i] +
+                (5000.0 / 7873.0) * dt * l1[i];
+    w7_rhs_1d(u2, n, a, dxinv, scheme, mlp, 0, face, l2);
+    for (int i = 0; i < n; ++i)
+        u3[i] = (53989.0 / 2500000.0) * u[i] - (102261.0 / 5000000.0) * dt * lt0[i] +
+                (4806213.0 / 20000000.0) * u1[i] - (5121.0 / 20000.0) * dt * lt1[i] + (23619.0 / 32000.0) * u2[i] +
+                (7873.0 / 10000.0) * dt * l2[i];
+    w7_rhs_1d(u3, n, a, dxinv, scheme, mlp, 0, face, l3);
+    for (int i = 0; i < n; ++i)
+        u[i] = (1.0 / 5.0) * u[i] + (1.0 / 10.0) * dt * l0[i] + (6127.0 / 30000.0) * u1[i] +
+               (1.0 / 6.0) * dt * l1[i] + (7873.0 / 30000.0) * u2[i] + (1.0 / 3.0) * u3[i] +
+               (1.0 / 6.0) * dt * l3[i];
+}
+
+int oracle_adv1d7_run(double *u, int batch, int n, const double *a, const double *dts, int nsteps, double dxinv,
+                      int scheme, const void *mlp) {
+    if (n < 7 || batch < 1) return -1;
+    if (scheme != SCHEME_JS && scheme != SCHEME_SR64 && scheme != SCHEME_SR32 && scheme != SCHEME_Z) return -2;
+    int rc = 0;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int b = 0; b < batch; ++b) {
+        double *w = (double *)malloc(sizeof(double) * 10 * (size_t)n);
+        if (!w) { rc = -3; continue; }
+        for (int s = 0; s < nsteps; ++s) w7_shu_rk4_1d(u + (size_t)b * n, n, a[b], dts[s], dxinv, scheme, mlp, w);
+        free(w);
+    }
+    return rc;
+}
+
+double oracle_face7_1d(const double *q, int lr, int scheme, const void *mlp, double h) {
+    return w7_face_1d(q, lr, scheme, mlp, h);
+}
+void oracle_features7(const double *q, double *f) { w7_features(q, f); }
+int oracle_plateau7(const double *q) { return w7_plateau(q); }
+void oracle_candidates7(const double *q, int lr, double *s) { w7_candidates(q, lr, s); }
+void oracle_classical_beta7(const double *q, double *b) { w7_classical_beta(q, b); }
+void oracle_mlp7_raw(const double *f, int n, int scheme, const void *mlp, double *out) {
+    for (int i = 0; i < n; ++i) w7_raw(f + 6 * i, scheme, mlp, out + 4 * i);
+}
+void oracle_mlp7_sym(const double *f, int n, int scheme, const void *mlp, double *out) {
+    for (int i = 0; i < n; ++i) w7_sym_ratios(f + 6 * i, scheme, mlp, out + 4 * i);
+}
+
+/* ---------------------------------------------------------------- 2-D path */
+/* WH7:557-564 weno7_beta: expanded quadratic forms (240 x the Jiang-Shu sums) */
+static void w7_beta_poly(const double q[7], double b[4]) {
+    double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6];
+    b[3] = 2107.0 * q3 * q3 - 9402.0 * q4 * q3 + 7042.0 * q3 * q5 - 1854.0 * q3 * q6 - 17246.0 * q4 * q5 -
+           3882.0 * q5 * q6 + 11003.0 * q4 * q4 + 7043.0 * q5 * q5 + 547.0 * q6 * q6 + 4642.0 * q4 * q6;
+    b[2] = 267.0 * q5 * q5 - 1642.0 * q5 * q4 + 1602.0 * q5 * q3 - 494.0 * q5 * q2 - 2522.0 * q3 * q2 +
+           3443.0 * q3 * q3 + 547.0 * q2 * q2 - 5966.0 * q4 * q3 + 2843.0 * q4 * q4 + 1922.0 * q4 * q2;
+    b[1] = 267.0 * q1 * q1 - 494.0 * q4 * q1 - 1642.0 * q2 * q1 + 1602.0 * q3 * q1 - 5966.0 * q3 * q2 +
+           3443.0 * q3 * q3 - 2522.0 * q4 * q3 + 2843.0 * q2 * q2 + 547.0 * q4 * q4 + 1922.0 * q4 * q2;
+    b[0] = 2107.0 * q3 * q3 + 7042.0 * q3 * q1 - 1854.0 * q3 * q0 - 9402.0 * q3 * q2 - 17246.0 * q2 * q1 +
+           4642.0 * q2 * q0 + 547.0 * q0 * q0 - 3882.0 * q1 * q0 + 11003.0 * q2 * q2 + 7043.0 * q1 * q1;
+}
+
+/* Z7:558-588 weno7_beta: the same quadratic forms as non-negative sums of squares */
+static void w7_beta_sos(const double q[7], double b[4]) {
+    double f[4][3];
+    w7_delta_forms(q, f);
+    const double c0 = 20.0 / 3.0, c1 = 260.0, c2 = 781.0 / 3.0;
+    for (int k = 0; k < 4; ++k) b[k] = c0 * f[k][0] * f[k][0] + c1 * f[k][1] * f[k][1] + c2 * f[k][2] * f[k][2];
+}
+
+/* Warp heads (1: x_{i+1/2}, 2: x_{i-1/2}; Gauss loca 1 / 2) -> the W7 heads
+ * with the same coefficients: the Gauss point of loca 1 is +sqrt(3)/6 (W7 head
+ * 4), of loca 2 -sqrt(3)/6 (W7 head 3). */
+static int w7_head(int gauss, int lr) { return gauss ? (lr == 1 ? 4 : 3) : lr; }
+
+/* BP:27-33 _mlp_beta: beta = 4 ratio, ones on a plateau; BP:171-200 the lr=2
+ * side evaluates the reversed stencil and reverses the result */
+static void w7_provider_beta(const double v[7], int side, int scheme, const void *mlp, double beta[4]) {
+    double q[7], b[4];
+    for (int m = 0; m < 7; ++m) q[m] = side == 0 ? v[m] : v[6 - m];
+    if (w7_plateau(q)) {
+        for (int k = 0; k < 4; ++k) b[k] = 1.0;
+    } else {
+        double f[6], r[4];
+        w7_features(q, f);
+        w7_sym_ratios(f, scheme, mlp, r);
+        for (int k = 0; k < 4; ++k) b[k] = 4.0 * r[k];
+    }
+    for (int k = 0; k < 4; ++k) beta[k] = side == 0 ? b[k] : b[3 - k];
+}
+
+/* WH7:741-776 weno7_lr_scalar (JS), EXT:92-120 external beta (SR), Z7:780-809 (Z) */
+static double w7_scalar_2d(const double q[7], int gauss, int lr, int scheme, const void *mlp, int eno, double h) {
+    const int head = w7_head(gauss, lr);
+    double s[4], d[4];
+    w7_candidates(q, head, s);
+    w7_optimal(head, d);
+    if (scheme == SCHEME_JS) {
+        double b[4], a[4];
+        w7_beta_poly(q, b);
+        for (int k = 0; k < 4; ++k) a[k] = d[k] * safe_rcp(b[k]) * safe_rcp(b[k]);
+        double asum = a[0] + a[1] + a[2] + a[3];
+        return (a[0] * s[0] + a[1] * s[1] + a[2] * s[2] + a[3] * s[3]) / asum;
+    }
+    if (scheme == SCHEME_Z) {
+        double b[4], a[4];
+        w7_beta_sos(q, b);
+        double tau7 = fabs(-b[0] - 3.0 * b[1] + 3.0 * b[2] + b[3]);
+        double eps_z = 240.0 * h * h * h;
+        for (int k = 0; k < 4; ++k) {
+            double r = tau7 / (b[k] + eps_z);
+            a[k] = d[k] * (1.0 + r * r * r);
+        }
+        double asum = a[0] + a[1] + a[2] + a[3];
+        return (a[0] * s[0] + a[1] * s[1] + a[2] * s[2] + a[3] * s[3]) / asum;
+    }
+    double beta[4], a[4], ww[4];
+    w7_provider_beta(q, lr == 1 ? 0 : 1, scheme, mlp, beta);
+    for (int k = 0; k < 4; ++k) {
+        double ib = 1.0 / (beta[k] + 1.0e-12);
+        a[k] = d[k] * ib * ib;
+    }
+    double asum = a[0] + a[1] + a[2] + a[3];
+    for (int k = 0; k < 4; ++k) ww[k] = a[k] / dmax(asum, 1.0e-300);
+    if (eno) {   /* EXT:46-62 */
+        double psi[4], cut = 0.0;
+        for (int k = 0; k < 4; ++k) psi[k] = ww[k] <= 4.0e-7 ? 0.0 : 1.0;
+        cut = psi[0] * ww[0] + psi[1] * ww[1] + psi[2] * ww[2] + psi[3] * ww[3];
+        double inv = 1.0 / dmax(cut, 1.0e-300);
+        for (int k = 0; k < 4; ++k) ww[k] = psi[k] * ww[k] * inv;
+    }
+    return ww[0] * s[0] + ww[1] * s[1] + ww[2] * s[2] + ww[3] * s[3];
+}
+
+/* WH7:256-342 evilin_state_2d (a GFORCE-type upwind state), WH7:429-433 */
+static vec4 evilin_state_2d(vec4 ul0, vec4 ur0, int direction, double c, double gamma) {
+    const double tiny = 1.0e-15;
+    vec4 wl0 = con_to_pri(ul0, gamma), wr0 = con_to_pri(ur0, gamma);
+    double un_l = direction == 2 ? wl0.v[2] : wl0.v[1];
+    double un_r = direction == 2 ? wr0.v[2] : wr0.v[1];
+    double a_l = sqrt(gamma * wl0.v[3] / wl0.v[0]);
+    double a_r = sqrt(gamma * wr0.v[3] / wr0.v[0]);
+    double smax = dmax(fabs(un_l) + a_l, fabs(un_r) + a_r);
+    smax = dmax(smax, tiny);
+    double dt_dx = c, dx_dt = 1.0 / dt_dx, local_cfl = c * smax;
+    vec4 fl = pri_to_flux(wl0, direction, gamma), fr = pri_to_flux(wr0, direction, gamma);
+    vec4 qlw, ffo, fgf, ulh, urh;
+    for (int k = 0; k < 4; ++k) qlw.v[k] = 0.5 * (ul0.v[k] + ur0.v[k]) - 0.5 * dt_dx * (fr.v[k] - fl.v[k]);
+    vec4 flw = pri_to_flux(con_to_pri(qlw, gamma), direction, gamma);
+    for (int k = 0; k < 4; ++k) ffo.v[k] = 0.25 * (fl.v[k] + 2.0 * flw.v[k] + fr.v[k] - dx_dt * (ur0.v[k] - ul0.v[k]));
+    double theta = (1.0 - local_cfl) / (1.0 + local_cfl);
+    for (int k = 0; k < 4; ++k) fgf.v[k] = theta * flw.v[k] + (1.0 - theta) * ffo.v[k];
+    for (int k = 0; k < 4; ++k) ulh.v[k] = ul0.v[k] - dt_dx * (fgf.v[k] - fl.v[k]);
+    for (int k = 0; k < 4; ++k) urh.v[k] = ur0.v[k] - dt_dx * (fr.v[k] - fgf.v[k]);
+    vec4 wl = con_to_pri(ulh, gamma), wr = con_to_pri(urh, gamma);
+    double rho_bar = dmax(0.5 * (wl.v[0] + wr.v[0]), tiny);
+    double p_bar = dmax(0.5 * (wl.v[3] + wr.v[3]), tiny);
+    double a_bar = dmax(sqrt(gamma * p_bar / rho_bar), tiny);
+    double c1 = rho_bar * a_bar, c2 = rho_bar / a_bar;
+    double un_le = direction == 2 ? wl.v[2] : wl.v[1];
+    double un_re = direction == 2 ? wr.v[2] : wr.v[1];
+    double u_star = 0.5 * (un_le + un_re) - 0.5 * (wr.v[3] - wl.v[3]) / c1;
+    double p_star = 0.5 * (wl.v[3] + wr.v[3]) - 0.5 * (un_re - un_le) * c1;
+    double rho_star_l = dmax(wl.v[0] + (un_le - u_star) * c2, tiny);
+    double rho_star_r = dmax(wr.v[0] + (u_star - un_re) * c2, tiny);
+    p_star = dmax(p_star, tiny);
+    double u_bar = 0.5 * (un_le + un_re);
+    double lam1 = u_bar - a_bar, lam2 = u_bar, lam3 = u_bar + a_bar;
+    vec4 wface = wl;
+    if (lam1 >= 0.0) wface = wl;
+    else if (lam2 >= 0.0) wface = direction == 1 ? v4(rho_star_l, u_star, wl.v[2], p_star) : v4(rho_star_l, wl.v[1], u_star, p_star);
+    else if (lam3 >= 0.0) wface = direction == 1 ? v4(rho_star_r, u_star, wr.v[2], p_star) : v4(rho_star_r, wr.v[1], u_star, p_star);
+    else wface = wr;
+    return pri_to_con(wface, gamma);
+}
+
+static vec4 w7_riemann(vec4 ul, vec4 ur, int direction, double c, double gamma, int riemann) {
+    if (riemann == RIEMANN7_HLLC) return hllc_flux(ul, ur, direction, gamma);
+    if (riemann == RIEMANN7_FORCE) return force_flux(ul, ur, direction, c, gamma);
+    return pri_to_flux(con_to_pri(evilin_state_2d(ul, ur, direction, c, gamma), gamma), direction, gamma);
+}
+
+typedef struct {
+    int scheme, eno_cutoff, riemann;
+    const void *mlp;
+    double gamma;
+} closure7_t;
+
+/* WH7:835-853 weno7_lr_vec_characteristic (EXT:168-204 for external beta):
+ * lr 1 uses Roe(q3, q4), lr 2 Roe(q2, q3) */
+static vec4 w7_char_face(const vec4 qs[7], int lr, int direction, const closure7_t *cl, double h) {
+    vec4 roe = lr == 1 ? roe_average_state(qs[3], qs[4], cl->gamma) : roe_average_state(qs[2], qs[3], cl->gamma);
+    vec4 c[7];
+    for (int m = 0; m < 7; ++m) c[m] = con_to_char(qs[m], roe, direction, cl->gamma);
+    vec4 face;
+    for (int k = 0; k < 4; ++k) {
+        double q[7];
+        for (int m = 0; m < 7; ++m) q[m] = c[m].v[k];
+        face.v[k] = w7_scalar_2d(q, 0, lr, cl->scheme, cl->mlp, cl->eno_cutoff, h);
+    }
+    return char_to_con(face, roe, direction, cl->gamma);
+}
+
+/* boundary fill of the WENO7 grid (ghost 4): WH7:2162-2200 (transmissive /
+ * periodic), DM7:63-127 (double Mach after a transmissive fill) */
+static void w7_fill_boundary(const double *src, double *dst, int nx, int ny, int boundary, double dx, double dy,
+                             double time, double gamma) {
+    const int gc = 4;
+    copy_boundary(src, dst, nx, ny, gc, boundary == BC_PERIODIC);
+    if (boundary == BC_DOUBLE_MACH) apply_double_mach(dst, nx, ny, gc, dx, dy, time, gamma);
+}
+
+/* P7:154-203 normal stages, P7:206-283 Gauss fluxes (reverse_upwind swaps the
+ * states), P7:286-302 rhs_from_flux; one RHS evaluation L(src) into rhs (the
+ * interior of rhs only).  work: 5 padded arrays + the four flux arrays. */
+static void w7_rhs_2d(const double *src, double *rhs, int nx, int ny, double dx, double dy, double dt, double time,
+                      int boundary, int reverse, const closure7_t *cl, double *work) {
+    const int gc = 4, nxt = nx + 2 * gc, nyt = ny + 2 * gc;
+    const size_t np = (size_t)nxt * nyt * 4;
+    double *bc = work, *xl = work + np, *xr = work + 2 * np, *yl = work + 3 * np, *yr = work + 4 * np;
+    double *fx1 = work + 5 * np, *fx2 = fx1 + (size_t)ny * (nx + 1) * 4;
+    double *fy1 = fx2 + (size_t)ny * (nx + 1) * 4, *fy2 = fy1 + (size_t)(ny + 1) * nx * 4;
+    w7_fill_boundary(src, bc, nx, ny, boundary, dx, dy, time, cl->gamma);
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int j = 1; j < ny + 7; ++j)
+        for (int i = 0; i < nx + 2; ++i) {
+            vec4 qs[7];
+            for (int m = 0; m < 7; ++m) qs[m] = load4(bc, j, i + m, nxt);
+            store4(xl, j, i, nxt, w7_char_face(qs, 2, 1, cl, dx));
+            store4(xr, j, i, nxt, w7_char_face(qs, 1, 1, cl, dx));
+        }
+#pragma omp parallel for schedule(dynamic, 2)
+    for (int j = 0; j < ny + 2; ++j)
+        for (int i = 1; i < nx + 7; ++i) {
+            vec4 qs[7];
+            for (int m = 0; m < 7; ++m) qs[m] = load4(bc, j + m, i, nxt);
+            store4(yl, j, i, nxt, w7_char_face(qs, 2, 2, cl, dy));
+            store4(yr, j, i, nxt, w7_char_face(qs, 1, 2, cl, dy));
+        }
+    for (int loca = 1; loca <= 2; ++loca) {
+        double *fx = loca == 1 ? fx1 : fx2, *fy = loca == 1 ? fy1 : fy2;
+#pragma omp parallel for schedule(dynamic, 2)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx + 1; ++i) {
+                vec4 ul, ur;
+                for (int k = 0; k < 4; ++k) {
+                    double ql[7], qr[7];
+                    for (int m = 0; m < 7; ++m) {
+                        ql[m] = xr[IDX(j + 1 + m, i, k, nxt)];
+                        qr[m] = xl[IDX(j + 1 + m, i + 1, k, nxt)];
+                    }
+                    ul.v[k] = w7_scalar_2d(ql, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dy);
+                    ur.v[k] = w7_scalar_2d(qr, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dy);
+                }
+                vec4 f = reverse ? w7_riemann(ur, ul, 1, dt / dx, cl->gamma, cl->riemann)
+                                 : w7_riemann(ul, ur, 1, dt / dx, cl->gamma, cl->riemann);
+                for (int k = 0; k < 4; ++k) fx[((size_t)j * (nx + 1) + i) * 4 + k] = f.v[k];
+            }
+#pragma omp parallel for schedule(dynamic, 2)
+        for (int j = 0; j < ny + 1; ++j)
+            for (int i = 0; i < nx; ++i) {
+                vec4 ul, ur;
+                for (int k = 0; k < 4; ++k) {
+                    double ql[7], qr[7];
+                    for (int m = 0; m < 7; ++m) {
+                        ql[m] = yr[IDX(j, i + 1 + m, k, nxt)];
+                        qr[m] = yl[IDX(j + 1, i + 1 + m, k, nxt)];
+                    }
+                    ul.v[k] = w7_scalar_2d(ql, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dx);
+                    ur.v[k] = w7_scalar_2d(qr, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dx);
+                }
+                vec4 f = reverse ? w7_riemann(ur, ul, 2, dt / dy, cl->gamma, cl->riemann)
+                                 : w7_riemann(ul, ur, 2, dt / dy, cl->gamma, cl->riemann);
+                for (int k = 0; k < 4; ++k) fy[((size_t)j * nx + i) * 4 + k] = f.v[k];
+            }
+    }
+    const double inv_dx = 1.0 / dx, inv_dy = 1.0 / dy;
+    memset(rhs, 0, np * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+            for (int k = 0; k < 4; ++k) {
+                double fxl = fx1[((size_t)j * (nx + 1) + i) * 4 + k] + fx2[((size_t)j * (nx + 1) + i) * 4 + k];
+                double fxr = fx1[((size_t)j * (nx + 1) + i + 1) * 4 + k] + fx2[((size_t)j * (nx + 1) + i + 1) * 4 + k];
+                double fyb = fy1[((size_t)j * nx + i) * 4 + k] + fy2[((size_t)j * nx + i) * 4 + k];
+                double fyt = fy1[((size_t)(j + 1) * nx + i) * 4 + k] + fy2[((size_t)(j + 1) * nx + i) * 4 + k];
+                rhs[IDX(j + gc, i + gc, k, nxt)] = -0.5 * (fxr - fxl) * inv_dx - 0.5 * (fyt - fyb) * inv_dy;
+            }
+}
+
+size_t oracle_euler7_work_doubles(int nx, int ny) {
+    size_t np = (size_t)(nx + 8) * (ny + 8) * 4;
+    return 5 * np + 2 * (size_t)ny * (nx + 1) * 4 + 2 * (size_t)(ny + 1) * nx * 4;
+}
+
+int oracle_euler7_rhs(const double *u, double *rhs, int nx, int ny, double dx, double dy, double gamma, double dt,
+                      double time, int scheme, int riemann, int boundary, int reverse, int eno_cutoff,
+                      const void *mlp) {
+    if ((scheme == SCHEME_SR64 || scheme == SCHEME_SR32) && !mlp) return -2;
+    double *work = (double *)malloc(sizeof(double) * oracle_euler7_work_doubles(nx, ny));
+    if (!work) return -3;
+    closure7_t cl = {scheme, eno_cutoff, riemann, mlp, gamma};
+    w7_rhs_2d(u, rhs, nx, ny, dx, dy, dt, time, boundary, reverse, &cl, work);
+    free(work);
+    return 0;
+}
+
+/* P7:314-404 / DM7:354-431 one Shu RK4 step of the padded state u (in place);
+ * boundary times t, t, t + dt/2, t + dt/2, t + dt/2, t + dt as in DM7 */
+int oracle_euler7_step(double *u, int nx, int ny, double dx, double dy, double gamma, double dt, double t,
+                       int scheme, int riemann, int boundary, int eno_cutoff, const void *mlp) {
+    if ((scheme == SCHEME_SR64 || scheme == SCHEME_SR32) && !mlp) return -2;
+    const size_t np = (size_t)(nx + 8) * (ny + 8) * 4;
+    double *work = (double *)malloc(sizeof(double) * (oracle_euler7_work_doubles(nx, ny) + 9 * np));
+    if (!work) return -3;
+    double *u1 = work + oracle_euler7_work_doubles(nx, ny), *u2 = u1 + np, *u3 = u2 + np;
+    double *r0 = u3 + np, *rt0 = r0 + np, *r1 = rt0 + np, *rt1 = r1 + np, *r2 = rt1 + np, *r3 = r2 + np;
+    closure7_t cl = {scheme, eno_cutoff, riemann, mlp, gamma};
+    w7_rhs_2d(u, r0, nx, ny, dx, dy, dt, t, boundary, 0, &cl, work);
+    for (size_t e = 0; e < np; ++e) u1[e] = u[e] + 0.5 * dt * r0[e];
+    w7_rhs_2d(u, rt0, nx, ny, dx, dy, dt, t, boundary, 1, &cl, work);
+    w7_rhs_2d(u1, r1, nx, ny, dx, dy, dt, t + 0.5 * dt, boundary, 0, &cl, work);
+    for (size_t e = 0; e < np; ++e)
+        u2[e] = 649.0 / 1600.0 * u[e] - 10890423.0 / 25193600.0 * dt * rt0[e] + 951.0 / 1600.0 * u1[e] +
+                5000.0 / 7873.0 * dt * r1[e];
+    w7_rhs_2d(u1, rt1, nx, ny, dx, dy, dt, t + 0.5 * dt, boundary, 1, &cl, work);
+    w7_rhs_2d(u2, r2, nx, ny, dx, dy, dt, t + 0.5 * dt, boundary, 0, &cl, work);
+    for (size_t e = 0; e < np; ++e)
+        u3[e] = 53989.0 / 2500000.0 * u[e] - 102261.0 / 5000000.0 * dt * rt0[e] + 4806213.0 / 20000000.0 * u1[e] -
+                5121.0 / 20000.0 * dt * rt1[e] + 23619.0 / 32000.0 * u2[e] + 7873.0 / 10000.0 * dt * r2[e];
+    w7_rhs_2d(u3, r3, nx, ny, dx, dy, dt, t + dt, boundary, 0, &cl, work);
+    for (size_t e = 0; e < np; ++e)
+        u[e] = 1.0 / 5.0 * u[e] + 1.0 / 10.0 * dt * r0[e] + 6127.0 / 30000.0 * u1[e] + 1.0 / 6.0 * dt * r1[e] +
+               7873.0 / 30000.0 * u2[e] + 1.0 / 3.0 * u3[e] + 1.0 / 6.0 * dt * r3[e];
+    free(work);
+    return 0;
+}
+
+/* Time loops.  mode 0 = P7:557-575 run_from_initial (dt clipped when t + dt
+ * passes t_end), mode 1 = DM7:453-470 run_to_time (stops at t_end - 1e-14,
+ * dt = min(dt_cfl, t_end - t), a non-finite dt ends the loop). */
+int oracle_euler7_run(double *u, int nx, int ny, double dx, double dy, double gamma, double cfl, double t_end,
+                      int scheme, int riemann, int boundary, int eno_cutoff, const void *mlp, int mode,
+                      int max_steps, double *dts, int *steps_out, double *t_out) {
+    double t = 0.0;
+    int step = 0;
+    while (step < max_steps) {
+        if (mode == 0 ? !(t < t_end) : !(t < t_end - 1.0e-14)) break;
+        double dt = oracle_euler2d_dt(u, nx, ny, 4, dx, dy, gamma, cfl);
+        if (mode == 0) {
+            if (t + dt > t_end) dt = t_end - t;
+            if (dt <= 0.0) break;
+        } else {
+            double rem = t_end - t;
+            dt = rem < dt ? rem : dt;
+            if (!isfinite(dt) || dt <= 0.0) break;
+        }
+        int rc = oracle_euler7_step(u, nx, ny, dx, dy, gamma, dt, t, scheme, riemann, boundary, eno_cutoff, mlp);
+        if (rc) return rc;
+        if (dts) dts[step] = dt;
+        t += dt;
+        step += 1;
+    }
+    *steps_out = step;
+    *t_out = t;
+    return 0;
+}
+
+void oracle_beta7_poly(const double *q, double *b) { w7_beta_poly(q, b); }
+void oracle_beta7_sos(const double *q, double *b) { w7_beta_sos(q, b); }
+double oracle_scalar7_2d(const double *q, int gauss, int lr, int scheme, const void *mlp, int eno, double h) {
+    return w7_scalar_2d(q, gauss, lr, scheme, mlp, eno, h);
+}
+void oracle_riemann7(const double *ul, const double *ur, int direction, double c, double gamma, int riemann,
+                     double *out) {
+    vec4 f = w7_riemann(v4(ul[0], ul[1], ul[2], ul[3]), v4(ur[0], ur[1], ur[2], ur[3]), direction, c, gamma, riemann);
+    memcpy(out, f.v, sizeof f.v);
+}
