@@ -72,8 +72,11 @@ def lib() -> ctypes.CDLL:
     return _lib
 
 
-def _ptr(a: np.ndarray | None) -> int | None:
-    return None if a is None else a.ctypes.data
+def _ptr(a: np.ndarray | None):
+    """Pointer argument that keeps the array alive for the duration of the call
+    (``data_as`` holds a reference, so temporaries such as a freshly packed
+    parameter block are not freed before the C function reads them)."""
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
 def _f64(a) -> np.ndarray:
@@ -278,3 +281,170 @@ def features_1d(q):
     f = np.zeros(5)
     _lib1d().oracle_features_1d(_ptr(q), _ptr(f))
     return f
+
+
+# ------------------------------------------------------------------ WENO7
+RIEMANN7 = {"hllc": 0, "force": 1, "evilin": 2}
+W7_NAMES = ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4")
+
+
+def pack_mlp7(mlp: dict | None, scheme: str) -> np.ndarray | None:
+    """Flatten the 6-24-16-16-4 network (908 parameters) in the oracle layout."""
+    if mlp is None or scheme in ("js", "z"):
+        return None
+    dtype = np.float64 if scheme == "sr64" else np.float32
+    flat = np.concatenate([np.asarray(mlp[n]).reshape(-1) for n in W7_NAMES]).astype(dtype)
+    if flat.size != 908:
+        raise ValueError(f"WENO7 MLP must have 908 parameters, got {flat.size}")
+    return np.ascontiguousarray(flat)
+
+
+def _lib7():
+    L = lib()
+    if not getattr(L, "_w7_ready", False):
+        L.oracle_adv1d7_run.restype = _I
+        L.oracle_adv1d7_run.argtypes = [_P, _I, _I, _P, _P, _I, _D, _I, _P]
+        L.oracle_face7_1d.restype = _D
+        L.oracle_face7_1d.argtypes = [_P, _I, _I, _P, _D]
+        L.oracle_features7.argtypes = [_P, _P]
+        L.oracle_optimal7.argtypes = [_I, _P]
+        L.oracle_plateau7.restype = _I
+        L.oracle_plateau7.argtypes = [_P]
+        L.oracle_candidates7.argtypes = [_P, _I, _P]
+        L.oracle_classical_beta7.argtypes = [_P, _P]
+        L.oracle_mlp7_raw.argtypes = [_P, _I, _I, _P, _P]
+        L.oracle_mlp7_sym.argtypes = [_P, _I, _I, _P, _P]
+        L.oracle_beta7_poly.argtypes = [_P, _P]
+        L.oracle_beta7_sos.argtypes = [_P, _P]
+        L.oracle_scalar7_2d.restype = _D
+        L.oracle_scalar7_2d.argtypes = [_P, _I, _I, _I, _P, _I, _D]
+        L.oracle_riemann7.argtypes = [_P, _P, _I, _D, _D, _I, _P]
+        L.oracle_euler7_rhs.restype = _I
+        L.oracle_euler7_rhs.argtypes = [_P, _P, _I, _I, _D, _D, _D, _D, _D, _I, _I, _I, _I, _I, _P]
+        L.oracle_euler7_step.restype = _I
+        L.oracle_euler7_step.argtypes = [_P, _I, _I, _D, _D, _D, _D, _D, _I, _I, _I, _I, _P]
+        L.oracle_euler7_run.restype = _I
+        L.oracle_euler7_run.argtypes = [_P, _I, _I, _D, _D, _D, _D, _D, _I, _I, _I, _I, _P, _I, _I, _P, _P, _P]
+        L._w7_ready = True
+    return L
+
+
+def adv1d7_run(u, velocities, dts, dxinv, scheme="js", mlp=None):
+    """Batch of periodic 1-D trajectories advanced by the signed Shu RK4 step
+    (rk4_advection.py:137-195) with WENO7 faces."""
+    u = np.array(np.atleast_2d(u), dtype=np.float64, order="C")
+    a = np.ascontiguousarray(np.broadcast_to(np.asarray(velocities, dtype=np.float64), (u.shape[0],)))
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    packed = pack_mlp7(mlp, scheme)
+    rc = _lib7().oracle_adv1d7_run(_ptr(u), u.shape[0], u.shape[1], _ptr(a), _ptr(dts), dts.size,
+                                   float(dxinv), SCHEMES[scheme], _ptr(packed))
+    if rc:
+        raise RuntimeError(f"oracle_adv1d7_run failed: {rc}")
+    return u
+
+
+def face7_1d(q, lr, scheme="js", mlp=None, h=1.0):
+    q = _f64(q)
+    return float(_lib7().oracle_face7_1d(_ptr(q), lr, SCHEMES[scheme], _ptr(pack_mlp7(mlp, scheme)), float(h)))
+
+
+def features7(q):
+    q = _f64(q)
+    f = np.zeros(6)
+    _lib7().oracle_features7(_ptr(q), _ptr(f))
+    return f
+
+
+def optimal7(lr):
+    d = np.zeros(4)
+    _lib7().oracle_optimal7(int(lr), _ptr(d))
+    return d
+
+
+def plateau7(q) -> bool:
+    return bool(_lib7().oracle_plateau7(_ptr(_f64(q))))
+
+
+def candidates7(q, lr):
+    s = np.zeros(4)
+    _lib7().oracle_candidates7(_ptr(_f64(q)), int(lr), _ptr(s))
+    return s
+
+
+def classical_beta7(q):
+    b = np.zeros(4)
+    _lib7().oracle_classical_beta7(_ptr(_f64(q)), _ptr(b))
+    return b
+
+
+def beta7_poly(q):
+    b = np.zeros(4)
+    _lib7().oracle_beta7_poly(_ptr(_f64(q)), _ptr(b))
+    return b
+
+
+def beta7_sos(q):
+    b = np.zeros(4)
+    _lib7().oracle_beta7_sos(_ptr(_f64(q)), _ptr(b))
+    return b
+
+
+def mlp7(feats, scheme, mlp, symmetric=True):
+    f = _f64(np.atleast_2d(feats))
+    out = np.zeros((f.shape[0], 4))
+    fn = _lib7().oracle_mlp7_sym if symmetric else _lib7().oracle_mlp7_raw
+    fn(_ptr(f), f.shape[0], SCHEMES[scheme], _ptr(pack_mlp7(mlp, scheme)), _ptr(out))
+    return out
+
+
+def scalar7_2d(q, gauss, lr, scheme="js", mlp=None, eno_cutoff=False, h=1.0):
+    return float(_lib7().oracle_scalar7_2d(_ptr(_f64(q)), int(gauss), int(lr), SCHEMES[scheme],
+                                           _ptr(pack_mlp7(mlp, scheme)), int(eno_cutoff), float(h)))
+
+
+def riemann7(ul, ur, direction, c, gamma=1.4, riemann="hllc"):
+    out = np.zeros(4)
+    _lib7().oracle_riemann7(_ptr(_f64(ul)), _ptr(_f64(ur)), direction, float(c), float(gamma), RIEMANN7[riemann],
+                            _ptr(out))
+    return out
+
+
+def rhs7_2d(u, nx, ny, dx, dy, gamma, dt, t=0.0, scheme="js", riemann="hllc", boundary="periodic",
+            reverse=False, eno_cutoff=False, mlp=None):
+    """One WENO7 right-hand side L(u) (ghost 4); the interior of the result."""
+    u = _f64(u)
+    rhs = np.zeros_like(u)
+    rc = _lib7().oracle_euler7_rhs(_ptr(u), _ptr(rhs), nx, ny, dx, dy, gamma, dt, t, SCHEMES[scheme],
+                                   RIEMANN7[riemann], BOUNDARY[boundary], int(reverse), int(eno_cutoff),
+                                   _ptr(pack_mlp7(mlp, scheme)))
+    if rc:
+        raise RuntimeError(f"oracle_euler7_rhs failed: {rc}")
+    return rhs
+
+
+def step7_2d(u, nx, ny, dx, dy, gamma, dt, t=0.0, scheme="js", riemann="hllc", boundary="periodic",
+             eno_cutoff=False, mlp=None):
+    u = np.array(u, dtype=np.float64, order="C")
+    rc = _lib7().oracle_euler7_step(_ptr(u), nx, ny, dx, dy, gamma, dt, t, SCHEMES[scheme], RIEMANN7[riemann],
+                                    BOUNDARY[boundary], int(eno_cutoff), _ptr(pack_mlp7(mlp, scheme)))
+    if rc:
+        raise RuntimeError(f"oracle_euler7_step failed: {rc}")
+    return u
+
+
+def run7_2d(u, nx, ny, dx, dy, gamma, cfl, t_end, scheme="js", riemann="hllc", boundary="periodic",
+            eno_cutoff=False, mlp=None, mode="point", max_steps=10_000_000):
+    """WENO7 + Shu RK4 time loop; mode "point" = point_rk4.run_from_initial,
+    "double-mach" = weno7_double_mach.run_to_time.  Returns (u, dts, steps, t)."""
+    u = np.array(u, dtype=np.float64, order="C")
+    cap = 200000 if max_steps > 200000 else max_steps
+    dts = np.zeros(cap)
+    steps = ctypes.c_int(0)
+    t = ctypes.c_double(0.0)
+    rc = _lib7().oracle_euler7_run(_ptr(u), nx, ny, dx, dy, gamma, cfl, t_end, SCHEMES[scheme], RIEMANN7[riemann],
+                                   BOUNDARY[boundary], int(eno_cutoff), _ptr(pack_mlp7(mlp, scheme)),
+                                   0 if mode == "point" else 1, min(max_steps, cap), _ptr(dts),
+                                   ctypes.byref(steps), ctypes.byref(t))
+    if rc:
+        raise RuntimeError(f"oracle_euler7_run failed: {rc}")
+    return u, dts[: steps.value].tolist(), steps.value, t.value

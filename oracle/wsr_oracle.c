@@ -1507,6 +1507,7 @@ double oracle_face7_1d(const double *q, int lr, int scheme, const void *mlp, dou
     return w7_face_1d(q, lr, scheme, mlp, h);
 }
 void oracle_features7(const double *q, double *f) { w7_features(q, f); }
+void oracle_optimal7(int lr, double *d) { w7_optimal(lr, d); }
 int oracle_plateau7(const double *q) { return w7_plateau(q); }
 void oracle_candidates7(const double *q, int lr, double *s) { w7_candidates(q, lr, s); }
 void oracle_classical_beta7(const double *q, double *b) { w7_classical_beta(q, b); }

@@ -47,3 +47,9 @@ def gpu_device():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session")
+def mlp7():
+    from weno_sr_publishing_amd.checkpoints import load_weno7_checkpoint, WENO7_SR_FP64
+    return load_weno7_checkpoint(WENO7_SR_FP64)

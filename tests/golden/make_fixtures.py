@@ -9,6 +9,7 @@ Sources (relative to the reference checkout):
   scripts/verify_release.py:157-222 (fixed feature vectors and expected MLP outputs)
   code/snapshot/for_paper_results/verification_hllc.json         (recorded HLLC agreement)
   experiments/01_gste_long_time_advection/tables/gste_metrics.csv (1-D GSTE metrics)
+  experiments/05_isentropic_vortex/tables/vortex_convergence.md (2-D vortex L1/L2, 9 digits)
   experiments/14_mixed_precision_timing/tables/weno5_precision_timing.tex (timings)
 """
 from __future__ import annotations
@@ -75,6 +76,18 @@ def gste() -> None:
         indent=2) + "\n")
 
 
+def vortex() -> None:
+    path = REF / "experiments/05_isentropic_vortex/tables/vortex_convergence.md"
+    rows = []
+    for line in path.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 6 and cells[1].isdigit():
+            rows.append({"method": cells[0], "N": int(cells[1]), "rho_l1": cells[2], "rho_l2": cells[4]})
+    (OUT / "vortex_convergence.json").write_text(json.dumps(
+        {"source": "experiments/05_isentropic_vortex/tables/vortex_convergence.md", "rows": rows},
+        indent=2) + "\n")
+
+
 def timing() -> None:
     path = REF / "experiments/14_mixed_precision_timing/tables/weno5_precision_timing.tex"
     (OUT / "timing_table.json").write_text(json.dumps(
@@ -88,5 +101,6 @@ if __name__ == "__main__":
     mlp_golden()
     hllc()
     gste()
+    vortex()
     timing()
     print("fixtures written to", OUT)

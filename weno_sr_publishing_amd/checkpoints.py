@@ -112,3 +112,78 @@ def save_mlp_params(path, params: dict, meta: dict | None = None) -> None:
         payload[name] = arr
     payload["meta_json"] = np.array(json.dumps(meta or {}, sort_keys=True))
     np.savez(path, **payload)
+
+
+# ------------------------------------------------------------------ WENO7 closure
+# code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/weno7_core.py:324-423:
+# w1 (1,6,h1) b1 (1,h1) w2 (1,h1,h2) b2 (1,h2) w3 (1,h2,h3) b3 (1,h3) w4 (1,h3,4) b4 (1,4)
+W7_INPUTS, W7_OUTPUTS, W7_HIDDEN = 6, 4, (24, 16, 16)
+MLP7_COUNT = 908
+
+
+def weno7_checkpoint_shapes(hidden=W7_HIDDEN) -> dict:
+    h1, h2, h3 = hidden
+    return {"w1": (1, W7_INPUTS, h1), "b1": (1, h1), "w2": (1, h1, h2), "b2": (1, h2),
+            "w3": (1, h2, h3), "b3": (1, h3), "w4": (1, h3, W7_OUTPUTS), "b4": (1, W7_OUTPUTS)}
+
+
+class Mlp7Params(dict):
+    """WENO7 closure parameters plus the packed 908-parameter block (order w1
+    (row-major), b1, ..., b4) in the closure dtype ("fp64" or "fp32")."""
+
+    def __init__(self, arrays: dict, precision: str, meta: dict | None = None):
+        super().__init__(arrays)
+        if precision not in ("fp64", "fp32"):
+            raise ValueError(f"unknown WENO7 closure precision {precision!r}")
+        self.precision = precision
+        self.meta = meta or {}
+        dtype = np.float64 if precision == "fp64" else np.float32
+        flat = np.concatenate([np.asarray(arrays[n]).reshape(-1) for n in REQUIRED]).astype(dtype)
+        if flat.size != MLP7_COUNT:
+            raise ValueError(f"WENO7 MLP must have {MLP7_COUNT} parameters (6-24-16-16-4), got {flat.size}")
+        self.packed = np.ascontiguousarray(flat)
+
+    def as_precision(self, precision: str) -> "Mlp7Params":
+        return Mlp7Params({k: np.asarray(v) for k, v in self.items()}, precision, self.meta)
+
+
+def load_weno7_checkpoint(path, device: str = "cuda", precision: str = "fp64") -> Mlp7Params:
+    """weno7_core.py:382-423 load_checkpoint: same required names, input / output /
+    hidden shape checks and error texts; read with ``allow_pickle=False``."""
+    del device
+    data = np.load(path, allow_pickle=False)
+    missing = [name for name in REQUIRED if name not in data.files]
+    if missing:
+        raise ValueError(f"{path} is missing arrays: {missing}")
+    if data["w1"].ndim != 3 or data["w1"].shape[1] != W7_INPUTS:
+        raise ValueError(f"{path} has incompatible input shape {data['w1'].shape}")
+    if data["w4"].ndim != 3 or data["w4"].shape[2] != W7_OUTPUTS:
+        raise ValueError(f"{path} has incompatible output shape {data['w4'].shape}")
+    hidden = (int(data["w1"].shape[2]), int(data["w2"].shape[2]), int(data["w3"].shape[2]))
+    expected = weno7_checkpoint_shapes(hidden)
+    wrong = {name: data[name].shape for name, shape in expected.items() if data[name].shape != shape}
+    if wrong:
+        raise ValueError(f"{path} has incompatible WENO7 MLP shapes: {wrong}")
+    if hidden != W7_HIDDEN:
+        raise ValueError(f"{path}: the GPU closure is built for hidden layers {W7_HIDDEN}, got {hidden}")
+    meta: dict = {}
+    if "meta_json" in data.files:
+        try:
+            meta = json.loads(str(data["meta_json"]))
+        except json.JSONDecodeError:
+            meta = {"raw": str(data["meta_json"])}
+    arrays = {name: np.array(data[name], dtype=np.float64) for name in REQUIRED}
+    return Mlp7Params(arrays, precision, meta)
+
+
+def save_weno7_checkpoint(path, params: dict, meta: dict | None = None) -> None:
+    """weno7_core.py:324-379 save_checkpoint layout (FP64, leading singleton axis,
+    ``meta_json`` with the architecture tag)."""
+    payload = {name: np.asarray(params[name], dtype=np.float64).reshape(shape)
+               for name, shape in weno7_checkpoint_shapes().items()}
+    full = dict(meta or {})
+    full.update({"mlp_architecture": "reflection_sym_direct_beta_ratio_6_24_16_16_4", "hidden": list(W7_HIDDEN),
+                 "precision": "float64", "reflection_formula": "0.5*(M(x)+P4*M(P6*x))",
+                 "deployment_requires_reflection_symmetrization": True})
+    payload["meta_json"] = np.array(json.dumps(full, sort_keys=True), dtype=np.str_)
+    np.savez(path, **payload)

@@ -6,6 +6,10 @@
 * Planar Sod strip at t=0: code/snapshot/teacherfree_lab_weno5/warp_sod_validation.py:48-62,176-210
   (15-point cell averages with the reference's own quadrature table).
 * ``pad_periodic``: code/snapshot/for_paper_results/run_vortex.py:34-35.
+* Periodic isentropic vortex on [-10, 10]^2 (WENO7 grid, ghost 4):
+  code/snapshot/weno7_point_rk4_shu/point_rk4.py:81-117 (``leggauss`` cell averages).
+* Double-Mach reflection on [0, 4] x [0, 1]: code/snapshot/run_double_mach_compare.py:43-88
+  (Mach-10 shock at 60 degrees, 15-point reference quadrature table).
 """
 from __future__ import annotations
 
@@ -175,3 +179,82 @@ def gste_cell_averages(nx: int, time: float, quadrature: int = 15) -> tuple[np.n
     samples = centers[:, None] + 0.5 * dx * xi[None, :] - time
     averages = 0.5 * np.sum(weights[None, :] * gste_point(samples), axis=1)
     return centers, averages
+
+
+# ---------------------------------------------------------------- isentropic vortex
+def _pdelta(d: np.ndarray, length: float) -> np.ndarray:
+    return d - length * np.round(d / length)
+
+
+def isentropic_vortex_conserved(x, y, t: float, gamma: float = 1.4) -> np.ndarray:
+    """point_rk4.py:81-99: vortex of strength 5 advected with (1, 1), period 20."""
+    beta, length = 5.0, 20.0
+    dx = _pdelta(x - 1.0 * t, length)
+    dy = _pdelta(y - 1.0 * t, length)
+    r2 = dx * dx + dy * dy
+    d_t = -((gamma - 1.0) * beta * beta) / (8.0 * gamma * np.pi * np.pi) * np.exp(1.0 - r2)
+    rho = 1.0 + (np.power(1.0 + d_t, 1.0 / (gamma - 1.0)) - 1.0)
+    p = 1.0 + (np.power(1.0 + d_t, gamma / (gamma - 1.0)) - 1.0)
+    factor = beta / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    u = 1.0 - factor * dy
+    v = 1.0 + factor * dx
+    return conserved_from_primitive(rho, u, v, p, gamma)
+
+
+def make_vortex_state(nx: int, ny: int, t: float = 0.0, quadrature: int = 15, ghost: int = 4,
+                      x_min: float = -10.0, x_max: float = 10.0, y_min: float = -10.0, y_max: float = 10.0,
+                      gamma: float = 1.4) -> np.ndarray:
+    """point_rk4.py:102-117 cell_average_state: tensor-product Gauss-Legendre
+    cell averages over the whole padded grid (ghost cells included)."""
+    xi, wi = np.polynomial.legendre.leggauss(quadrature)
+    dx = (x_max - x_min) / float(nx)
+    dy = (y_max - y_min) / float(ny)
+    jj, ii = np.indices((ny + 2 * ghost, nx + 2 * ghost))
+    xc = x_min + (ii - ghost + 0.5) * dx
+    yc = y_min + (jj - ghost + 0.5) * dy
+    out = np.zeros((ny + 2 * ghost, nx + 2 * ghost, 4), dtype=np.float64)
+    for sx, wx in zip(xi.astype(np.float64), wi.astype(np.float64)):
+        x = xc + 0.5 * dx * float(sx)
+        for sy, wy in zip(xi.astype(np.float64), wi.astype(np.float64)):
+            y = yc + 0.5 * dy * float(sy)
+            out += 0.25 * float(wx) * float(wy) * isentropic_vortex_conserved(x, y, t, gamma)
+    return out
+
+
+# ---------------------------------------------------------------- double-Mach reflection
+DOUBLE_MACH_POST = (8.0, 8.25 * np.sqrt(3.0) * 0.5, -8.25 * 0.5, 116.5)
+DOUBLE_MACH_PRE = (1.4, 0.0, 0.0, 1.0)
+
+
+def double_mach_conserved(x, y, t: float, gamma: float = 1.4) -> np.ndarray:
+    """run_double_mach_compare.py:43-49: post-shock state left of x0 + (y + 20 t)/sqrt(3)."""
+    root3 = np.sqrt(3.0)
+    shock_x = 1.0 / 6.0 + y / root3 + 20.0 * t / root3
+    post = conserved_from_primitive(*DOUBLE_MACH_POST, gamma)
+    pre = conserved_from_primitive(*DOUBLE_MACH_PRE, gamma)
+    left = (np.asarray(x) < shock_x)[..., None]
+    return np.where(left, post, pre)
+
+
+def make_double_mach_state(nx: int = 1200, ny: int = 300, ghost: int = 4, quadrature: int = 15,
+                           x_length: float = 4.0, y_length: float = 1.0, gamma: float = 1.4) -> np.ndarray:
+    """run_double_mach_compare.py:69-88 make_initial_state: cell averages (the
+    reference's 15-point table) over the padded grid, origin at (0, 0)."""
+    if quadrature == 15:
+        xi, wi = GAUSS15_XI, GAUSS15_W
+    elif quadrature == 1:
+        xi, wi = np.array([0.0]), np.array([2.0])
+    else:
+        raise ValueError("quadrature must be 1 or 15")
+    dx = x_length / nx
+    dy = y_length / ny
+    jj, ii = np.indices((ny + 2 * ghost, nx + 2 * ghost))
+    xc = (ii - ghost + 0.5) * dx
+    yc = (jj - ghost + 0.5) * dy
+    state = np.zeros((ny + 2 * ghost, nx + 2 * ghost, 4), dtype=np.float64)
+    for ax, wx in zip(xi, wi):
+        x = xc + 0.5 * dx * float(ax)
+        for ay, wy in zip(xi, wi):
+            y = yc + 0.5 * dy * float(ay)
+            state += float(wx) * float(wy) * double_mach_conserved(x, y, 0.0, gamma)
+    return 0.25 * state
