@@ -32,7 +32,9 @@ enum {
 /* Riemann fluxes */
 enum {
     WSR_RIEMANN_HLLC = 0,  /* code/snapshot/for_paper_results/solvers/weno5_hllc.py:23-115 */
-    WSR_RIEMANN_FORCE = 1  /* code/snapshot/warp_weno5_helpers.py:401-435 */
+    WSR_RIEMANN_FORCE = 1, /* code/snapshot/warp_weno5_helpers.py:401-435 */
+    WSR_RIEMANN_EVILIN = 2 /* WENO7 only: code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/
+                              warp_weno7_ader4_helpers_classical_only.py:256-342,429-433 */
 };
 /* boundary conditions */
 enum {
@@ -51,7 +53,9 @@ enum {
 
 /* instrumentation flags (wsr_euler2d_config.flags) */
 enum {
-    WSR_FLAG_COUNT_CLOSURE = 1  /* count reconstructions and closure (MLP) evaluations into the workspace */
+    WSR_FLAG_COUNT_CLOSURE = 1, /* count reconstructions and closure (MLP) evaluations into the workspace */
+    WSR_FLAG_Z_FALLBACK = 2     /* WENO7-SR: a closure ratio that is not finite or outside [0, 1] switches that
+                                   reconstruction to WENO7-Z (no reference counterpart) */
 };
 
 /* Grid and method description (a plain C struct; mirrors the fields of
@@ -167,6 +171,58 @@ int wsr_adv1d_train_adjoint(const wsr_adv1d_config *cfg, const double *checkpoin
                             double *lam_initial, void *stream);
 
 /* Library version / build identification. */
+/* ------------------------------------------------------------------------ */
+/* 2-D WENO7 with Shu's downwind fourth-order TVD Runge-Kutta (ghost 4)      */
+/* ------------------------------------------------------------------------ */
+/* time-loop rules */
+enum {
+    WSR_LOOP7_POINT = 0,       /* weno7_point_rk4_shu/point_rk4.py:557-575 run_from_initial */
+    WSR_LOOP7_DOUBLE_MACH = 1  /* for_paper_results/solvers/weno7_double_mach.py:434-470 run_to_time */
+};
+
+/* Mirrors point_rk4.py:26-55 Params (nx, ny, ghost = 4, gamma, cfl, t_end,
+ * dx = x_length / nx) plus the method switches.  scheme: WSR_SCHEME_JS
+ * (WENO7-JS, warp helpers' expanded indicators), WSR_SCHEME_SR64 / _SR32
+ * (WENO7-SR, 6-24-16-16-4 closure in FP64 / FP32), WSR_SCHEME_Z (WENO7-Z p=3);
+ * mlp = the 908 parameters w1, b1, ..., b4 in the checkpoint order (FP64 for
+ * SR64, FP32 for SR32). */
+typedef struct wsr_euler7_config {
+    int32_t nx, ny, ghost;
+    int32_t scheme, riemann, boundary;
+    int32_t eno_cutoff;
+    int32_t flags;
+    int32_t loop_mode;
+    int32_t reserved;
+    double dx, dy, gamma;
+    double cfl, t_end;
+    const void *mlp;
+} wsr_euler7_config;
+
+size_t wsr_euler7_workspace_bytes(const wsr_euler7_config *cfg);
+
+/* L(u) into rhs and, when rhs_reverse != NULL, the downwind operator L~(u)
+ * into rhs_reverse (padded arrays, ghost cells set to 0); replaces
+ * point_rk4.py:410-468 compute_rhs (reverse_upwind 0 / 1) and
+ * point_rk4_mlp.py:232-304 compute_rhs_mlp.  dt enters FORCE / EVILIN only;
+ * time is the double-Mach boundary time. */
+int wsr_euler7_rhs(const wsr_euler7_config *cfg, const double *u, double *rhs, double *rhs_reverse, double dt,
+                   double time, void *workspace, size_t workspace_bytes, void *stream);
+
+/* One Shu RK4 step of u in place (point_rk4.py:471-493 launch_shu_rk4_step,
+ * point_rk4_mlp.py:307-327, weno7_double_mach.py:354-431). */
+int wsr_euler7_step(const wsr_euler7_config *cfg, double *u, double dt, double time, void *workspace,
+                    size_t workspace_bytes, void *stream);
+
+/* Device-resident time loop (replaces the host loops of point_rk4.py:537-575
+ * and weno7_double_mach.py:434-470): loop_init resets the clock and reduces
+ * the max speed of u; loop_steps enqueues n_steps steps (dt on the device,
+ * dts[step] = dt); wsr_euler2d_loop_status / wsr_euler2d_counters read the
+ * same control block. */
+int wsr_euler7_loop_init(const wsr_euler7_config *cfg, const double *u, int32_t max_steps, void *workspace,
+                         size_t workspace_bytes, void *stream);
+int wsr_euler7_loop_steps(const wsr_euler7_config *cfg, double *u, double *dts, int32_t n_steps, void *workspace,
+                          size_t workspace_bytes, void *stream);
+
 const char *wsr_version(void);
 
 #ifdef __cplusplus

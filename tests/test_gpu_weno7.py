@@ -1,0 +1,128 @@
+"""The HIP WENO7 step (wsr_euler7_*) against the CPU oracle and the reference's
+recorded vortex errors.
+
+Tolerances: FP64 closures (JS, SR64, Z) 1e-12 relative (max-norm of the
+interior); FP32 closure 2e-6 relative for one right-hand side.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"js": 1e-12, "sr64": 1e-12, "z": 1e-12, "sr32": 2e-6}
+
+
+def rich_state7(nx, ny, seed=0, gamma=1.4):
+    rng = np.random.default_rng(seed)
+    x = (np.arange(-4, nx + 4) + 0.5) / nx
+    y = (np.arange(-4, ny + 4) + 0.5) / ny
+    X, Y = np.meshgrid(x, y)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * X + 0.4) * np.cos(2 * np.pi * Y) + 0.6 * (X + 0.7 * Y > 0.8)
+    rho = np.where((X < 0.25) & (Y < 0.3), 0.8, rho)
+    u = 0.4 * np.cos(2 * np.pi * Y) + 0.2 * (X > 0.6)
+    v = -0.3 * np.sin(2 * np.pi * X) + 0.1 * rng.normal(size=X.shape) * (Y > 0.5)
+    p = 1.0 + 0.2 * np.cos(2 * np.pi * (X - Y)) + 0.5 * (Y > 0.4)
+    u = np.where((X < 0.25) & (Y < 0.3), 0.0, u)
+    v = np.where((X < 0.25) & (Y < 0.3), 0.0, v)
+    p = np.where((X < 0.25) & (Y < 0.3), 1.0, p)
+    E = p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+    return np.ascontiguousarray(np.stack([rho, rho * u, rho * v, E], axis=-1))
+
+
+def rel(a, b, g=4):
+    a, b = a[g:-g, g:-g], b[g:-g, g:-g]
+    return float(np.max(np.abs(a - b)) / max(1e-300, np.max(np.abs(b))))
+
+
+def solver(params, scheme, mlp7, boundary="periodic", riemann="hllc", loop="point"):
+    from weno_sr_publishing_amd.euler7 import Euler7Solver
+    m = None
+    if scheme in ("sr64", "sr32"):
+        m = mlp7 if scheme == "sr64" else mlp7.as_precision("fp32")
+    return Euler7Solver(params, scheme, riemann, boundary, m, loop=loop)
+
+
+@pytest.mark.parametrize("scheme", ["js", "sr64", "sr32", "z"])
+@pytest.mark.parametrize("boundary,riemann", [("periodic", "hllc"), ("transmissive", "force"),
+                                              ("double-mach", "evilin")])
+def test_rhs7_parity(oracle, mlp7, gpu_device, scheme, boundary, riemann):
+    from weno_sr_publishing_amd.euler7 import Params7
+    nx, ny = 37, 29
+    p = Params7(nx=nx, ny=ny, x_min=0.0, x_max=1.0, y_min=0.0, y_max=1.0)
+    u0 = rich_state7(nx, ny, 1)
+    s = solver(p, scheme, mlp7, boundary, riemann)
+    u = s.new_state(u0)
+    m = None if scheme in ("js", "z") else (mlp7 if scheme == "sr64" else mlp7.as_precision("fp32"))
+    for reverse in (False, True):
+        got = s.rhs(u, dt=0.002, time=0.01, reverse=reverse).cpu().numpy()
+        ref = oracle.rhs7_2d(u0, nx, ny, p.dx, p.dy, 1.4, 0.002, 0.01, scheme=scheme, riemann=riemann,
+                             boundary=boundary, reverse=reverse, mlp=m)
+        err = rel(got, ref)
+        print(f"weno7 rhs parity {scheme} {boundary} {riemann} reverse={reverse}: {err:.3e}")
+        assert err <= TOL[scheme]
+        assert np.all(got[:4] == 0.0) and np.all(got[:, :4] == 0.0)
+
+
+@pytest.mark.parametrize("scheme", ["js", "sr64", "sr32"])
+def test_step7_parity(oracle, mlp7, gpu_device, scheme):
+    from weno_sr_publishing_amd.euler7 import Params7
+    nx, ny = 40, 33
+    p = Params7(nx=nx, ny=ny, x_min=0.0, x_max=1.0, y_min=0.0, y_max=1.0)
+    u0 = rich_state7(nx, ny, 2)
+    s = solver(p, scheme, mlp7, "transmissive")
+    u = s.new_state(u0)
+    m = None if scheme == "js" else (mlp7 if scheme == "sr64" else mlp7.as_precision("fp32"))
+    ref = u0.copy()
+    for n in range(3):
+        s.step(u, 0.001, 0.001 * n)
+        ref = oracle.step7_2d(ref, nx, ny, p.dx, p.dy, 1.4, 0.001, 0.001 * n, scheme=scheme,
+                              boundary="transmissive", mlp=m)
+    err = rel(u.cpu().numpy(), ref)
+    print(f"weno7 step parity {scheme}: {err:.3e}")
+    assert err <= (1e-12 if scheme != "sr32" else 1e-6)
+
+
+@pytest.mark.parametrize("scheme", ["js", "sr64"])
+def test_vortex7_device_loop_matches_reference_table(mlp7, gpu_device, scheme):
+    """N=25 and 50 periodic vortex to t=2 (device dt, point_rk4 loop) against the
+    reference's recorded L1 / L2 density errors (9 digits)."""
+    from weno_sr_publishing_amd.euler7 import Params7
+    from weno_sr_publishing_amd.initial import make_vortex_state
+    method = {"js": "WENO7-JS-RK4", "sr64": "WENO7-SR-RK4"}[scheme]
+    for n in (25, 50):
+        ref = [r for r in golden("vortex_convergence.json")["rows"] if r["method"] == method and r["N"] == n][0]
+        p = Params7(nx=n, ny=n, cfl=0.4, t_end=2.0)
+        s = solver(p, scheme, mlp7, "periodic", "hllc", "point")
+        u = s.new_state(make_vortex_state(n, n, 0.0))
+        dts, steps, t = s.run(u, 2.0, chunk=8)
+        exact = make_vortex_state(n, n, 2.0)
+        d = u.cpu().numpy()[4:-4, 4:-4, 0] - exact[4:-4, 4:-4, 0]
+        l1, l2 = float(np.mean(np.abs(d))), float(np.sqrt(np.mean(d * d)))
+        print(f"vortex {scheme} N={n}: steps {steps} t {t!r} L1 {l1:.9e} (ref {ref['rho_l1']}) L2 {l2:.9e}")
+        assert abs(t - 2.0) <= 1e-12
+        assert abs(l1 / float(ref["rho_l1"]) - 1.0) <= 5e-9
+        assert abs(l2 / float(ref["rho_l2"]) - 1.0) <= 5e-9
+
+
+def test_double_mach7_loop_matches_oracle(oracle, mlp7, gpu_device):
+    """Coarse double-Mach reflection (120 x 30), WENO7-SR FP64 + HLLC, the
+    weno7_double_mach run_to_time rule, 4 steps: device loop vs oracle loop."""
+    from weno_sr_publishing_amd.euler7 import double_mach_params7
+    from weno_sr_publishing_amd.initial import make_double_mach_state
+    p = double_mach_params7(120, 30, t_end=0.2)
+    u0 = make_double_mach_state(120, 30)
+    s = solver(p, "sr64", mlp7, "double-mach", "hllc", "double-mach")
+    u = s.new_state(u0)
+    dts, steps, t = s.run(u, 0.2, max_steps=4)
+    ref, rdts, rsteps, rt = oracle.run7_2d(u0, 120, 30, p.dx, p.dy, 1.4, 0.4, 0.2, scheme="sr64", riemann="hllc",
+                                           boundary="double-mach", mlp=mlp7, mode="double-mach", max_steps=4)
+    assert steps == rsteps == 4
+    np.testing.assert_allclose(dts, rdts, rtol=1e-12)
+    err = rel(u.cpu().numpy(), ref)
+    print(f"double-mach weno7 sr64 loop parity: {err:.3e}")
+    assert err <= 1e-11

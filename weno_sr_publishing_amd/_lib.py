@@ -11,7 +11,7 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent / "libwsr_hip.so"
 
 SCHEMES = {"js": 0, "sr64": 1, "sr32": 2, "srbf16": 3, "z": 4}
-RIEMANN = {"hllc": 0, "force": 1}
+RIEMANN = {"hllc": 0, "force": 1, "evilin": 2}
 BOUNDARY = {"transmissive": 0, "periodic": 1, "double-mach": 2}
 ERRORS = {-1: "bad argument", -2: "SR scheme without MLP parameters", -3: "workspace too small",
           -4: "HIP runtime error"}
@@ -21,8 +21,12 @@ EXPORTS = (
     "wsr_euler2d_loop_init", "wsr_euler2d_loop_steps", "wsr_euler2d_loop_status", "wsr_version",
     "wsr_euler2d_loop_steps_timed", "wsr_euler2d_counters", "wsr_adv1d_rollout",
     "wsr_adv1d_train_forward", "wsr_adv1d_train_adjoint",
+    "wsr_euler7_workspace_bytes", "wsr_euler7_rhs", "wsr_euler7_step", "wsr_euler7_loop_init",
+    "wsr_euler7_loop_steps",
 )
 FLAG_COUNT_CLOSURE = 1
+FLAG_Z_FALLBACK = 2
+LOOP7 = {"point": 0, "double-mach": 1}
 
 
 class Euler2DConfig(ctypes.Structure):
@@ -31,6 +35,19 @@ class Euler2DConfig(ctypes.Structure):
         ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ghost", ctypes.c_int32),
         ("scheme", ctypes.c_int32), ("riemann", ctypes.c_int32), ("boundary", ctypes.c_int32),
         ("eno_cutoff", ctypes.c_int32), ("flags", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("mlp", ctypes.c_void_p),
+    ]
+
+
+class Euler7Config(ctypes.Structure):
+    """Mirror of ``wsr_euler7_config``."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("ghost", ctypes.c_int32),
+        ("scheme", ctypes.c_int32), ("riemann", ctypes.c_int32), ("boundary", ctypes.c_int32),
+        ("eno_cutoff", ctypes.c_int32), ("flags", ctypes.c_int32),
+        ("loop_mode", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("gamma", ctypes.c_double),
         ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
         ("mlp", ctypes.c_void_p),
@@ -92,6 +109,17 @@ def lib() -> ctypes.CDLL:
         L.wsr_adv1d_train_forward.argtypes = [A1, P, P, P, I32, P, P, P]
         L.wsr_adv1d_train_adjoint.restype = ctypes.c_int
         L.wsr_adv1d_train_adjoint.argtypes = [A1, P, P, P, P, I32, P, P, P, P]
+        C7 = ctypes.POINTER(Euler7Config)
+        L.wsr_euler7_workspace_bytes.restype = SZ
+        L.wsr_euler7_workspace_bytes.argtypes = [C7]
+        L.wsr_euler7_rhs.restype = ctypes.c_int
+        L.wsr_euler7_rhs.argtypes = [C7, P, P, P, D, D, P, SZ, P]
+        L.wsr_euler7_step.restype = ctypes.c_int
+        L.wsr_euler7_step.argtypes = [C7, P, D, D, P, SZ, P]
+        L.wsr_euler7_loop_init.restype = ctypes.c_int
+        L.wsr_euler7_loop_init.argtypes = [C7, P, I32, P, SZ, P]
+        L.wsr_euler7_loop_steps.restype = ctypes.c_int
+        L.wsr_euler7_loop_steps.argtypes = [C7, P, P, I32, P, SZ, P]
         L.wsr_version.restype = ctypes.c_char_p
         _lib = L
     return _lib

@@ -34,6 +34,7 @@
 
 #include "../../include/wsr.h"
 #include "wsr_math.hpp"
+#include "wsr_loop.hpp"
 
 namespace wsr {
 
@@ -50,37 +51,7 @@ constexpr int NT1 = TT + 4;
 constexpr int NN = TN + 5;         // stencil cells per transverse line in a tile
 constexpr int SBLOCK = NT1 * TN;   // sweep threads: one face-line of phase 1 each
 static_assert(SBLOCK % 64 == 0 && SBLOCK <= 1024, "sweep block must be whole waves");
-constexpr int NCOUNT = 6;
-
-struct Ctl {
-    double t, dt, t_end, cfl_h, dt_last;
-    unsigned long long speed_bits;
-    int32_t step, done, nan_flag, max_steps;
-    // closure evaluations (WSR_FLAG_COUNT_CLOSURE): [0] normal reconstructions,
-    // [1] of which evaluated the closure (not a plateau), [2] Gauss-pair
-    // reconstructions, [3] of which evaluated the closure, [4] / [5] lanes of
-    // the waves that ran the closure code (normal / Gauss; 64 per wave pass)
-    unsigned long long counters[NCOUNT];
-};
-
-struct Grid {
-    int nx, ny, nxt, nyt;
-    double dx, dy, gamma;
-};
-
 // ---------------------------------------------------------------- boundary
-__device__ __forceinline__ void dm_exact(double x, double y, double t, double gamma, double q[4]) {
-    // weno5_hllc_refsym.py:25-54 double_mach_exact_conserved
-    const double root3 = 1.7320508075688772;
-    double x0 = 1.0 / 6.0;
-    double shock_x = x0 + y / root3 + 20.0 * t / root3;
-    double rho, u, v, p;
-    if (x < shock_x) { rho = 8.0; u = 8.25 * root3 * 0.5; v = -8.25 * 0.5; p = 116.5; }
-    else { rho = 1.4; u = 0.0; v = 0.0; p = 1.0; }
-    double e = 0.5 * rho * (u * u + v * v) + p / (gamma - 1.0);
-    q[0] = rho; q[1] = rho * u; q[2] = rho * v; q[3] = e;
-}
-
 __global__ __launch_bounds__(BLOCK) void k_boundary(const double *__restrict__ src, double *__restrict__ dst,
                                                     Grid g, int boundary, double time_host,
                                                     const Ctl *ctl, double time_frac) {
@@ -142,24 +113,6 @@ struct SweepArgs {
 };
 
 // wave-level count of lanes with `flag` set, added to *dst by one lane
-// per-lane closure counts, summed over the wave and added once per wave at the end
-struct LaneCounts {
-    unsigned c[NCOUNT] = {0u, 0u, 0u, 0u, 0u, 0u};
-    __device__ __forceinline__ void add(int slot, int wave_slot, bool counted, bool evaluated) {
-        c[slot] += counted ? 1u : 0u;
-        c[slot + 1] += (counted && evaluated) ? 1u : 0u;
-        c[wave_slot] += __ballot(evaluated) != 0ull ? 1u : 0u;
-    }
-    __device__ __forceinline__ void flush(unsigned long long *dst) {
-#pragma unroll
-        for (int i = 0; i < NCOUNT; ++i) {
-            unsigned v = c[i];
-            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
-            if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst + i, (unsigned long long)v);
-        }
-    }
-};
-
 template <int DIR>
 __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
     if (DIR == 1) { fl = it % TN; tau = it / TN; }      // faces fastest (x: contiguous cells)
@@ -315,41 +268,6 @@ __global__ __launch_bounds__(SBLOCK) void k_sweep(const double *__restrict__ bc,
 }
 
 // ------------------------------------------------------------------ update
-__device__ __forceinline__ double cell_speed(const double q[4], double gamma) {
-    // warp_weno5_helpers.py:876-897 compute_max_speed_kernel
-    double rho = dmax(q[0], 1.0e-15);
-    double vx = q[1] / rho, vy = q[2] / rho;
-    double p = (gamma - 1.0) * (q[3] - 0.5 * rho * (vx * vx + vy * vy));
-    p = dmax(p, 1.0e-15);
-    double c = sqrt(gamma * p / rho);
-    return dmax(fabs(vx) + c, fabs(vy) + c);
-}
-
-// block max of non-negative speeds (NaN reported separately) -> atomics in ctl
-__device__ __forceinline__ void reduce_speed(double sp, bool active, Ctl *ctl) {
-    __shared__ unsigned long long red[BLOCK / 64];
-    __shared__ int nan_seen;
-    if (threadIdx.x == 0) nan_seen = 0;
-    __syncthreads();
-    unsigned long long bits = 0ull;
-    if (active) {
-        if (sp != sp) nan_seen = 1;
-        else bits = (unsigned long long)__double_as_longlong(sp);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_down(bits, off);
-        bits = o > bits ? o : bits;
-    }
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = bits;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long m = red[0];
-        for (int w = 1; w < BLOCK / 64; ++w) m = red[w] > m ? red[w] : m;
-        atomicMax(&ctl->speed_bits, m);
-        if (nan_seen) atomicOr(&ctl->nan_flag, 1);
-    }
-}
-
 __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ us, const double *u0,
                                                   double *u_out, const double *__restrict__ Fx,
                                                   const double *__restrict__ Fy, Grid g, int rk,
@@ -376,7 +294,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ us,
         for (int comp = 0; comp < 4; ++comp) u_out[c + comp] = q[comp];
     }
     if (fold_speed) {
-        reduce_speed(active ? cell_speed(q, g.gamma) : 0.0, active, ctl);
+        reduce_speed<BLOCK>(active ? cell_speed(q, g.gamma) : 0.0, active, ctl);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             // run_to_time bookkeeping: t += dt; step += 1; dts.append(dt)
             double dt = ctl->dt;
@@ -398,7 +316,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_speed(const double *__restrict__ 
         double q[4] = {c[0], c[1], c[2], c[3]};
         sp = cell_speed(q, g.gamma);
     }
-    reduce_speed(sp, active, ctl);
+    reduce_speed<BLOCK>(sp, active, ctl);
 }
 
 __global__ void k_reset_speed(Ctl *ctl) {
@@ -439,8 +357,6 @@ struct Workspace {
     double *bc, *Fx, *Fy;
 };
 
-static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
-
 static size_t ws_layout(int nx, int ny, Workspace *w, void *base) {
     size_t nstate = (size_t)(nx + 6) * (ny + 6) * 4 * sizeof(double);
     size_t nfx = (size_t)ny * (nx + 1) * 4 * sizeof(double);
@@ -460,15 +376,6 @@ static size_t ws_layout(int nx, int ny, Workspace *w, void *base) {
     return off;
 }
 
-static float host_round_bf16(float x) {
-    uint32_t u;
-    memcpy(&u, &x, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) return x;
-    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
-    float y;
-    memcpy(&y, &u, 4);
-    return y;
-}
 
 static int check_cfg(const wsr_euler2d_config *cfg) {
     if (!cfg) return WSR_E_ARG;

@@ -1,0 +1,432 @@
+// wsr_weno7.hpp -- device arithmetic of the WENO7 step (gfx950): candidate
+// polynomials, smoothness indicators, the 6-24-16-16-4 closure and the EVILIN
+// state.
+//
+// Reference files (paths relative to the reference checkout):
+//   W7  = code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/weno7_core.py
+//   WH7 = code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/
+//         warp_weno7_ader4_helpers_classical_only.py
+//   EXT = code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/
+//         warp_weno7_external_overwrite_fullall.py
+//   BP  = code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/beta_provider.py
+//   Z7  = code/snapshot/weno_z_borges_p2_results/weno7_helpers_z.py
+#pragma once
+
+#include "wsr_math.hpp"
+
+namespace wsr {
+
+constexpr int MLP7_COUNT = 908;
+// reference layout: w1[6][24] b1[24] w2[24][16] b2[16] w3[16][16] b3[16] w4[16][4] b4[4]
+constexpr int O7_W1 = 0, O7_B1 = 144, O7_W2 = 168, O7_B2 = 552, O7_W3 = 568, O7_B3 = 824, O7_W4 = 840,
+              O7_B4 = 904;
+// device layout, neuron-major runs [b_o, w_0o, ..., w_(K-1)o]
+constexpr int D7_L1 = 0, D7_L2 = 168, D7_L3 = 568, D7_L4 = 840;
+
+template <int SCHEME> struct Mlp7Sweep { using type = MlpNone; };
+template <> struct Mlp7Sweep<SCHEME_SR64> { using type = MlpK<double>; };
+template <> struct Mlp7Sweep<SCHEME_SR32> { using type = MlpK<float>; };
+
+template <class T>
+__host__ __device__ inline void pack7_neuron_major(const T *ref, T *dev) {
+    const int K[4] = {6, 24, 16, 16}, N[4] = {24, 16, 16, 4};
+    const int W[4] = {O7_W1, O7_W2, O7_W3, O7_W4}, B[4] = {O7_B1, O7_B2, O7_B3, O7_B4};
+    int o = 0;
+    for (int l = 0; l < 4; ++l)
+        for (int n = 0; n < N[l]; ++n) {
+            dev[o++] = ref[B[l] + n];
+            for (int k = 0; k < K[l]; ++k) dev[o++] = ref[W[l] + k * N[l] + n];
+        }
+}
+
+// W7:28-128 candidate coefficients and optimal weights of the four heads
+// (1: x_{i+1/2}, 2: x_{i-1/2}, 3: Gauss point -sqrt(3)/6, 4: +sqrt(3)/6), as the
+// double values of the reference's expressions.  WH7's Gauss location 1..4
+// (loca 1) are head 4, locations 5..8 (loca 2) head 3.
+template <int HEAD>
+__device__ __forceinline__ void candidates7(const double q[7], double s[4]) {
+    constexpr double C[4][4][4] = {
+        {{-0.25, 1.0833333333333333, -1.9166666666666667, 2.0833333333333335},
+         {0.08333333333333333, -0.4166666666666667, 1.0833333333333333, 0.25},
+         {-0.08333333333333333, 0.5833333333333334, 0.5833333333333334, -0.08333333333333333},
+         {0.25, 1.0833333333333333, -0.4166666666666667, 0.08333333333333333}},
+        {{0.08333333333333333, -0.4166666666666667, 1.0833333333333333, 0.25},
+         {-0.08333333333333333, 0.5833333333333334, 0.5833333333333334, -0.08333333333333333},
+         {0.25, 1.0833333333333333, -0.4166666666666667, 0.08333333333333333},
+         {2.0833333333333335, -1.9166666666666667, 1.0833333333333333, -0.25}},
+        {{0.08820629112619283, -0.4089564406759849, 0.8419691425682042, 0.4787810069815879},
+         {-0.056131276171213614, 0.31273139581104725, 0.8316061714863592, -0.08820629112619283},
+         {0.08820629112619283, 1.1683938285136408, -0.31273139581104725, 0.056131276171213614},
+         {1.5212189930184121, -0.8419691425682042, 0.4089564406759849, -0.08820629112619283}},
+        {{-0.08820629112619283, 0.4089564406759849, -0.8419691425682042, 1.5212189930184121},
+         {0.056131276171213614, -0.31273139581104725, 1.1683938285136408, 0.08820629112619283},
+         {-0.08820629112619283, 0.8316061714863592, 0.31273139581104725, -0.056131276171213614},
+         {0.4787810069815879, 0.8419691425682042, -0.4089564406759849, 0.08820629112619283}}};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        s[k] = q[k] * C[HEAD - 1][k][0] + q[k + 1] * C[HEAD - 1][k][1] + q[k + 2] * C[HEAD - 1][k][2] +
+               q[k + 3] * C[HEAD - 1][k][3];
+}
+
+template <int HEAD>
+__device__ __forceinline__ void optimal7(double d[4]) {
+    constexpr double D[4][4] = {
+        {0.02857142857142857, 0.34285714285714286, 0.5142857142857142, 0.11428571428571428},
+        {0.11428571428571428, 0.5142857142857142, 0.34285714285714286, 0.02857142857142857},
+        {0.06756615284017824, 0.4381942580088793, 0.42771483290021167, 0.06652475625073086},
+        {0.06652475625073086, 0.42771483290021167, 0.4381942580088793, 0.06756615284017824}};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) d[k] = D[HEAD - 1][k];
+}
+
+// W7:139-163 the twelve difference forms of the four sub-stencils
+__device__ __forceinline__ void delta_forms7(const double q[7], double f[4][3]) {
+    f[0][0] = -2.0 * q[0] + 9.0 * q[1] - 18.0 * q[2] + 11.0 * q[3];
+    f[0][1] = -q[0] + 4.0 * q[1] - 5.0 * q[2] + 2.0 * q[3];
+    f[0][2] = -q[0] + 3.0 * q[1] - 3.0 * q[2] + q[3];
+    f[1][0] = q[1] - 6.0 * q[2] + 3.0 * q[3] + 2.0 * q[4];
+    f[1][1] = q[2] - 2.0 * q[3] + q[4];
+    f[1][2] = -q[1] + 3.0 * q[2] - 3.0 * q[3] + q[4];
+    f[2][0] = -2.0 * q[2] - 3.0 * q[3] + 6.0 * q[4] - q[5];
+    f[2][1] = q[2] - 2.0 * q[3] + q[4];
+    f[2][2] = -q[2] + 3.0 * q[3] - 3.0 * q[4] + q[5];
+    f[3][0] = -11.0 * q[3] + 18.0 * q[4] - 9.0 * q[5] + 2.0 * q[6];
+    f[3][1] = 2.0 * q[3] - 5.0 * q[4] + 4.0 * q[5] - q[6];
+    f[3][2] = -q[3] + 3.0 * q[4] - 3.0 * q[5] + q[6];
+}
+
+// WH7:557-564 weno7_beta (expanded integer quadratic forms, 240 x Jiang-Shu)
+__device__ __forceinline__ void beta7_poly(const double q[7], double b[4]) {
+    const double q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4], q5 = q[5], q6 = q[6];
+    b[3] = 2107.0 * q3 * q3 - 9402.0 * q4 * q3 + 7042.0 * q3 * q5 - 1854.0 * q3 * q6 - 17246.0 * q4 * q5 -
+           3882.0 * q5 * q6 + 11003.0 * q4 * q4 + 7043.0 * q5 * q5 + 547.0 * q6 * q6 + 4642.0 * q4 * q6;
+    b[2] = 267.0 * q5 * q5 - 1642.0 * q5 * q4 + 1602.0 * q5 * q3 - 494.0 * q5 * q2 - 2522.0 * q3 * q2 +
+           3443.0 * q3 * q3 + 547.0 * q2 * q2 - 5966.0 * q4 * q3 + 2843.0 * q4 * q4 + 1922.0 * q4 * q2;
+    b[1] = 267.0 * q1 * q1 - 494.0 * q4 * q1 - 1642.0 * q2 * q1 + 1602.0 * q3 * q1 - 5966.0 * q3 * q2 +
+           3443.0 * q3 * q3 - 2522.0 * q4 * q3 + 2843.0 * q2 * q2 + 547.0 * q4 * q4 + 1922.0 * q4 * q2;
+    b[0] = 2107.0 * q3 * q3 + 7042.0 * q3 * q1 - 1854.0 * q3 * q0 - 9402.0 * q3 * q2 - 17246.0 * q2 * q1 +
+           4642.0 * q2 * q0 + 547.0 * q0 * q0 - 3882.0 * q1 * q0 + 11003.0 * q2 * q2 + 7043.0 * q1 * q1;
+}
+
+// Z7:558-588 the same indicators as non-negative sums of squares
+__device__ __forceinline__ void beta7_sos(const double q[7], double b[4]) {
+    double f[4][3];
+    delta_forms7(q, f);
+    const double c0 = 20.0 / 3.0, c1 = 260.0, c2 = 781.0 / 3.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) b[k] = c0 * f[k][0] * f[k][0] + c1 * f[k][1] * f[k][1] + c2 * f[k][2] * f[k][2];
+}
+
+// W7:166-221 plateau test and the six closure inputs (torch semantics: scale
+// channel (log10 + 16) / 16 without a remap); false = plateau
+__device__ __forceinline__ bool features7(const double q[7], double f[6]) {
+    double fm[4][3];
+    delta_forms7(q, fm);
+    const double c0 = 1.0 / 36.0, c1 = 13.0 / 12.0, c2 = 781.0 / 720.0;
+    double delta[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) delta[k] = fabs(fm[k][0]) * c0 + fabs(fm[k][1]) * c1 + fabs(fm[k][2]) * c2;
+    double dm = dmax(dmax(dmax(delta[0], delta[1]), delta[2]), delta[3]);
+    double qs = fabs(q[0]);
+#pragma unroll
+    for (int m = 1; m < 7; ++m) qs = dmax(qs, fabs(q[m]));
+    qs = dmax(qs, 1.0);
+    if (dm <= 1.0e-13 * qs) return false;
+    double inv = 1.0 / dmax(dm, 1.0e-15);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) f[k] = delta[k] * inv;
+    double g = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double d2 = q[i] - 2.0 * q[i + 1] + q[i + 2];
+        double den = fabs(q[i + 1] - q[i]) + fabs(q[i + 2] - q[i + 1]) + 1.0e-15;
+        double r = fabs(d2) / den;
+        g = i == 0 ? r : dmax(g, r);
+    }
+    f[4] = dmin(dmax(g, 0.0), 1.0);
+    double rel = dmax(dm / qs, 1.0e-30);
+    f[5] = dmin(dmax((log10(rel) + 16.0) / 16.0, 0.0), 1.0);
+    return true;
+}
+
+// ----------------------------------------------------------- the closure
+// W7:301-311: three swish layers, softmax(6 tanh(z / 6)) over 4 outputs; the
+// reflection average 0.5 [M(x) + P4 M(P6 x)] with P6 reversing the deltas.
+__device__ __forceinline__ void softmax4_64(const double z[4], double r[4]) {
+    double b[4];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) b[o] = 6.0 * tanh(z[o] / 6.0);
+    double m = dmax(dmax(dmax(b[0], b[1]), b[2]), b[3]);
+    double e[4], s = 0.0;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) { e[o] = exp(b[o] - m); s = s + e[o]; }
+    double inv = 1.0 / s;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) r[o] = e[o] * inv;
+}
+
+__device__ __forceinline__ void softmax4_32(const float z[4], float r[4]) {
+    const float sixth = 1.0f / 6.0f;
+    float b[4];
+#pragma unroll
+    for (int o = 0; o < 4; ++o) b[o] = 6.0f * tanh_fast(z[o] * sixth);
+    float m = fmax_(fmax_(fmax_(b[0], b[1]), b[2]), b[3]);
+    float e[4], s = 0.0f;
+#pragma unroll
+    for (int o = 0; o < 4; ++o) { e[o] = exp_fast(b[o] - m); s = s + e[o]; }
+    float inv = rcp_fast(s);
+#pragma unroll
+    for (int o = 0; o < 4; ++o) r[o] = e[o] * inv;
+}
+
+// one dense layer for a pair of inputs, one neuron's weights per scalar fetch
+template <int K, int N, class T, class ACT>
+__device__ __forceinline__ void dense7_pair(const T *xa, const T *xb, const T *w, int base, T *ha, T *hb, ACT act) {
+#pragma unroll
+    for (int o = 0; o < N; ++o) {
+        kconst_ptr<T> c = kparams(w + base + o * (K + 1));
+        T za = c[0], zb = c[0];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            za = za + xa[k] * c[1 + k];
+            zb = zb + xb[k] * c[1 + k];
+        }
+        ha[o] = act(za);
+        hb[o] = act(zb);
+    }
+}
+
+// ratios of the reflection-averaged closure for the six inputs f (FP64 or FP32)
+template <int SCHEME>
+__device__ __forceinline__ void sym_ratios7(const double f[6], const typename Mlp7Sweep<SCHEME>::type &P,
+                                            double r[4]) {
+    if constexpr (SCHEME == SCHEME_SR64) {
+        double xa[6] = {f[0], f[1], f[2], f[3], f[4], f[5]};
+        double xb[6] = {f[3], f[2], f[1], f[0], f[4], f[5]};
+        double a1[24], b1[24], a2[16], b2[16], a3[16], b3[16], za[4], zb[4], ra[4], rb[4];
+        auto sw = [](double z) { return swish64(z); };
+        auto id = [](double z) { return z; };
+        dense7_pair<6, 24>(xa, xb, P.p, D7_L1, a1, b1, sw);
+        dense7_pair<24, 16>(a1, b1, P.p, D7_L2, a2, b2, sw);
+        dense7_pair<16, 16>(a2, b2, P.p, D7_L3, a3, b3, sw);
+        dense7_pair<16, 4>(a3, b3, P.p, D7_L4, za, zb, id);
+        softmax4_64(za, ra);
+        softmax4_64(zb, rb);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = 0.5 * (ra[k] + rb[3 - k]);
+    } else {
+        float xa[6] = {(float)f[0], (float)f[1], (float)f[2], (float)f[3], (float)f[4], (float)f[5]};
+        float xb[6] = {xa[3], xa[2], xa[1], xa[0], xa[4], xa[5]};
+        float a1[24], b1[24], a2[16], b2[16], a3[16], b3[16], za[4], zb[4], ra[4], rb[4];
+        auto sw = [](float z) { return swish32(z); };
+        auto id = [](float z) { return z; };
+        dense7_pair<6, 24>(xa, xb, P.p, D7_L1, a1, b1, sw);
+        dense7_pair<24, 16>(a1, b1, P.p, D7_L2, a2, b2, sw);
+        dense7_pair<16, 16>(a2, b2, P.p, D7_L3, a3, b3, sw);
+        dense7_pair<16, 4>(a3, b3, P.p, D7_L4, za, zb, id);
+        softmax4_32(za, ra);
+        softmax4_32(zb, rb);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = 0.5 * ((double)ra[k] + (double)rb[3 - k]);
+    }
+}
+
+// BP:27-33 beta = 4 r, ones on a plateau.  `reversed` evaluates the mirrored
+// stencil and mirrors the result (BP:185-200, the lr = 2 side); with the
+// reflection-averaged closure this equals the direct evaluation up to rounding.
+// Returns false for a plateau.  With `fallback`, a non-finite or out-of-range
+// ratio (outside [0, 1]) sets `invalid`.
+template <int SCHEME>
+__device__ __forceinline__ bool closure_beta7(const double v[7], bool reversed,
+                                              const typename Mlp7Sweep<SCHEME>::type &P, double beta[4],
+                                              bool &invalid) {
+    double q[7], f[6], r[4];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) q[m] = reversed ? v[6 - m] : v[m];
+    invalid = false;
+    if (!features7(q, f)) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) beta[k] = 1.0;
+        return false;
+    }
+    sym_ratios7<SCHEME>(f, P, r);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double rk = reversed ? r[3 - k] : r[k];
+        invalid = invalid || !(rk >= 0.0 && rk <= 1.0);
+        beta[k] = 4.0 * rk;
+    }
+    return true;
+}
+
+// EXT:64-79 weights from beta: a = d / (beta + 1e-12)^2, normalised (+ ENO cutoff)
+__device__ __forceinline__ double combine_beta7(const double s[4], const double d[4], const double beta[4], int eno) {
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double ib = 1.0 / (beta[k] + 1.0e-12);
+        a[k] = d[k] * ib * ib;
+    }
+    double inv = 1.0 / dmax(a[0] + a[1] + a[2] + a[3], 1.0e-300);
+    double w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = a[k] * inv;
+    if (eno) {
+        double p[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p[k] = w[k] <= 4.0e-7 ? 0.0 : 1.0;
+        double ic = 1.0 / dmax(p[0] * w[0] + p[1] * w[1] + p[2] * w[2] + p[3] * w[3], 1.0e-300);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = p[k] * w[k] * ic;
+    }
+    return w[0] * s[0] + w[1] * s[1] + w[2] * s[2] + w[3] * s[3];
+}
+
+// WH7:741-776 (JS): a = d / (1e-6 + beta)^2 with the expanded indicators
+__device__ __forceinline__ double combine_js7(const double s[4], const double d[4], const double b[4]) {
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double r = 1.0 / (1.0e-6 + b[k]);
+        a[k] = d[k] * r * r;
+    }
+    double asum = a[0] + a[1] + a[2] + a[3];
+    return (a[0] * s[0] + a[1] * s[1] + a[2] * s[2] + a[3] * s[3]) / asum;
+}
+
+// Z7:780-809 (WENO-Z, p = 3, tau_7 of Castro-Costa-Don, eps = 240 h^3)
+__device__ __forceinline__ double combine_z7(const double s[4], const double d[4], const double b[4], double h) {
+    double tau7 = fabs(-b[0] - 3.0 * b[1] + 3.0 * b[2] + b[3]);
+    double eps_z = 240.0 * h * h * h;
+    double a[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double r = tau7 / (b[k] + eps_z);
+        a[k] = d[k] * (1.0 + r * r * r);
+    }
+    double asum = a[0] + a[1] + a[2] + a[3];
+    return (a[0] * s[0] + a[1] * s[1] + a[2] * s[2] + a[3] * s[3]) / asum;
+}
+
+// Normal face value of one characteristic component: lr 1 (HEAD 1) or lr 2 (HEAD 2).
+template <int SCHEME, int HEAD>
+__device__ __forceinline__ double recon7_normal(const double q[7], const typename Mlp7Sweep<SCHEME>::type &P,
+                                                int eno, double h, bool z_fallback, bool &closure, bool &fell_back) {
+    double s[4], d[4], b[4];
+    candidates7<HEAD>(q, s);
+    optimal7<HEAD>(d);
+    closure = false;
+    fell_back = false;
+    if constexpr (SCHEME == SCHEME_JS) {
+        beta7_poly(q, b);
+        return combine_js7(s, d, b);
+    } else if constexpr (SCHEME == SCHEME_Z) {
+        beta7_sos(q, b);
+        return combine_z7(s, d, b, h);
+    } else {
+        bool invalid;
+        closure = closure_beta7<SCHEME>(q, HEAD == 2, P, b, invalid);
+        if (z_fallback && invalid) {
+            fell_back = true;
+            beta7_sos(q, b);
+            return combine_z7(s, d, b, h);
+        }
+        return combine_beta7(s, d, b, eno);
+    }
+}
+
+// Both transverse Gauss-point values (loca 1 = head 4, loca 2 = head 3) of one
+// conservative component; one closure evaluation serves both points (the
+// reference evaluates the mirrored stencil for loca 2, equal up to rounding).
+template <int SCHEME>
+__device__ __forceinline__ void recon7_gauss(const double q[7], const typename Mlp7Sweep<SCHEME>::type &P, int eno,
+                                             double h, bool z_fallback, double &g1, double &g2, bool &closure,
+                                             bool &fell_back) {
+    double s1[4], s2[4], d1[4], d2[4], b[4];
+    candidates7<4>(q, s1);
+    candidates7<3>(q, s2);
+    optimal7<4>(d1);
+    optimal7<3>(d2);
+    closure = false;
+    fell_back = false;
+    if constexpr (SCHEME == SCHEME_JS) {
+        beta7_poly(q, b);
+        g1 = combine_js7(s1, d1, b);
+        g2 = combine_js7(s2, d2, b);
+    } else if constexpr (SCHEME == SCHEME_Z) {
+        beta7_sos(q, b);
+        g1 = combine_z7(s1, d1, b, h);
+        g2 = combine_z7(s2, d2, b, h);
+    } else {
+        bool invalid;
+        closure = closure_beta7<SCHEME>(q, false, P, b, invalid);
+        if (z_fallback && invalid) {
+            fell_back = true;
+            beta7_sos(q, b);
+            g1 = combine_z7(s1, d1, b, h);
+            g2 = combine_z7(s2, d2, b, h);
+        } else {
+            g1 = combine_beta7(s1, d1, b, eno);
+            g2 = combine_beta7(s2, d2, b, eno);
+        }
+    }
+}
+
+// WH7:256-342 evilin_state_2d; the flux is f(evilin state) (WH7:429-433)
+template <int DIR>
+__device__ void evilin_flux(const double ul0[4], const double ur0[4], double c, double gamma, double out[4]) {
+    const double tiny = 1.0e-15;
+    double wl0[4], wr0[4];
+    con_to_pri(ul0, gamma, wl0);
+    con_to_pri(ur0, gamma, wr0);
+    const int n = DIR == 1 ? 1 : 2;
+    double a_l = sqrt(gamma * wl0[3] / wl0[0]), a_r = sqrt(gamma * wr0[3] / wr0[0]);
+    double smax = dmax(dmax(fabs(wl0[n]) + a_l, fabs(wr0[n]) + a_r), tiny);
+    double dt_dx = c, dx_dt = 1.0 / dt_dx, local_cfl = c * smax;
+    double fl[4], fr[4], qlw[4], wlw[4], flw[4], fgf[4], ulh[4], urh[4];
+    pri_to_flux<DIR>(wl0, gamma, fl);
+    pri_to_flux<DIR>(wr0, gamma, fr);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) qlw[k] = 0.5 * (ul0[k] + ur0[k]) - 0.5 * dt_dx * (fr[k] - fl[k]);
+    con_to_pri(qlw, gamma, wlw);
+    pri_to_flux<DIR>(wlw, gamma, flw);
+    double theta = (1.0 - local_cfl) / (1.0 + local_cfl);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double ffo = 0.25 * (fl[k] + 2.0 * flw[k] + fr[k] - dx_dt * (ur0[k] - ul0[k]));
+        fgf[k] = theta * flw[k] + (1.0 - theta) * ffo;
+        ulh[k] = ul0[k] - dt_dx * (fgf[k] - fl[k]);
+        urh[k] = ur0[k] - dt_dx * (fr[k] - fgf[k]);
+    }
+    double wl[4], wr[4];
+    con_to_pri(ulh, gamma, wl);
+    con_to_pri(urh, gamma, wr);
+    double rho_bar = dmax(0.5 * (wl[0] + wr[0]), tiny);
+    double p_bar = dmax(0.5 * (wl[3] + wr[3]), tiny);
+    double a_bar = dmax(sqrt(gamma * p_bar / rho_bar), tiny);
+    double c1 = rho_bar * a_bar, c2 = rho_bar / a_bar;
+    double un_le = wl[n], un_re = wr[n];
+    double u_star = 0.5 * (un_le + un_re) - 0.5 * (wr[3] - wl[3]) / c1;
+    double p_star = 0.5 * (wl[3] + wr[3]) - 0.5 * (un_re - un_le) * c1;
+    double rho_star_l = dmax(wl[0] + (un_le - u_star) * c2, tiny);
+    double rho_star_r = dmax(wr[0] + (u_star - un_re) * c2, tiny);
+    p_star = dmax(p_star, tiny);
+    double u_bar = 0.5 * (un_le + un_re);
+    double w[4];
+    if (u_bar - a_bar >= 0.0) {
+        w[0] = wl[0]; w[1] = wl[1]; w[2] = wl[2]; w[3] = wl[3];
+    } else if (u_bar >= 0.0) {
+        w[0] = rho_star_l; w[1] = wl[1]; w[2] = wl[2]; w[3] = p_star; w[n] = u_star;
+    } else if (u_bar + a_bar >= 0.0) {
+        w[0] = rho_star_r; w[1] = wr[1]; w[2] = wr[2]; w[3] = p_star; w[n] = u_star;
+    } else {
+        w[0] = wr[0]; w[1] = wr[1]; w[2] = wr[2]; w[3] = wr[3];
+    }
+    // pri_to_con then con_to_pri (WH7:432) before the physical flux
+    double e = 0.5 * w[0] * (w[1] * w[1] + w[2] * w[2]) + w[3] / (gamma - 1.0);
+    double qf[4] = {w[0], w[0] * w[1], w[0] * w[2], e}, wf[4];
+    con_to_pri(qf, gamma, wf);
+    pri_to_flux<DIR>(wf, gamma, out);
+}
+
+}  // namespace wsr
