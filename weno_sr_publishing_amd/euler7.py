@@ -82,7 +82,10 @@ class Euler7Solver:
         if loop not in _lib.LOOP7:
             raise ValueError(f"unknown time loop {loop!r}")
         self.params, self.scheme, self.riemann, self.boundary = params, scheme, riemann, boundary
-        self.device = torch.device(device)
+        dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self._mlp_host = None
         if scheme in ("sr64", "sr32"):
             if mlp is None:
