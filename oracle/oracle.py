@@ -448,3 +448,63 @@ def run7_2d(u, nx, ny, dx, dy, gamma, cfl, t_end, scheme="js", riemann="hllc", b
     if rc:
         raise RuntimeError(f"oracle_euler7_run failed: {rc}")
     return u, dts[: steps.value].tolist(), steps.value, t.value
+
+
+# ------------------------------------------------------------------ 3-D WENO5
+def _lib3():
+    L = lib()
+    if not getattr(L, "_w3_ready", False):
+        L.oracle_euler3_work_doubles.restype = ctypes.c_size_t
+        L.oracle_euler3_work_doubles.argtypes = [_I, _I, _I]
+        L.oracle_euler3_step.restype = _I
+        L.oracle_euler3_step.argtypes = [_P, _I, _I, _I, _D, _D, _D, _D, _I, _I, _I, _P]
+        L.oracle_euler3_max_speed.restype = _D
+        L.oracle_euler3_max_speed.argtypes = [_P, _I, _I, _I]
+        L.oracle_euler3_run.restype = _I
+        L.oracle_euler3_run.argtypes = [_P, _I, _I, _I, _D, _D, _D, _D, _D, _I, _I, _I, _I, _P, _P, _P, _P]
+        L.oracle_flux3.argtypes = [_P, _P, _I, _D, _I, _P]
+        L.oracle_eigen3.argtypes = [_P, _I, _P, _P]
+        L._w3_ready = True
+    return L
+
+
+def step3d(q, nx, ny, nz, dx, dy, dz, dt, scheme="js", riemann="evilin", eno_cutoff=False, mlp=None):
+    """One SSPRK3 step of the padded (nz+6, ny+6, nx+6, 5) state (solver.py:166-210)."""
+    q = np.array(q, dtype=np.float64, order="C")
+    rc = _lib3().oracle_euler3_step(_ptr(q), nx, ny, nz, dx, dy, dz, dt, SCHEMES[scheme], RIEMANN7[riemann],
+                                    int(eno_cutoff), _ptr(pack_mlp(mlp, scheme)))
+    if rc:
+        raise RuntimeError(f"oracle_euler3_step failed: {rc}")
+    return q
+
+
+def max_speed3d(q, nx, ny, nz) -> float:
+    return float(_lib3().oracle_euler3_max_speed(_ptr(_f64(q)), nx, ny, nz))
+
+
+def run3d(q, nx, ny, nz, dx, dy, dz, cfl, t_end, max_steps=10_000_000, scheme="js", riemann="evilin",
+          eno_cutoff=False, mlp=None):
+    q = np.array(q, dtype=np.float64, order="C")
+    cap = min(max_steps, 100000)
+    dts = np.zeros(cap)
+    steps = ctypes.c_int(0)
+    t = ctypes.c_double(0.0)
+    rc = _lib3().oracle_euler3_run(_ptr(q), nx, ny, nz, dx, dy, dz, cfl, t_end, cap, SCHEMES[scheme],
+                                   RIEMANN7[riemann], int(eno_cutoff), _ptr(pack_mlp(mlp, scheme)), _ptr(dts),
+                                   ctypes.byref(steps), ctypes.byref(t))
+    if rc:
+        raise RuntimeError(f"oracle_euler3_run failed: {rc}")
+    return q, dts[: steps.value].tolist(), steps.value, t.value
+
+
+def flux3d(ul, ur, direction, dt_h, riemann="evilin"):
+    out = np.zeros(5)
+    _lib3().oracle_flux3(_ptr(_f64(ul)), _ptr(_f64(ur)), direction, float(dt_h), RIEMANN7[riemann], _ptr(out))
+    return out
+
+
+def eigen3d(state, direction):
+    L = np.zeros((5, 5))
+    R = np.zeros((5, 5))
+    _lib3().oracle_eigen3(_ptr(_f64(state)), direction, _ptr(L), _ptr(R))
+    return L, R

@@ -1848,3 +1848,412 @@ void oracle_riemann7(const double *ul, const double *ur, int direction, double c
     vec4 f = w7_riemann(v4(ul[0], ul[1], ul[2], ul[3]), v4(ur[0], ur[1], ur[2], ur[3]), direction, c, gamma, riemann);
     memcpy(out, f.v, sizeof f.v);
 }
+
+/* ======================================================================== */
+/* 3-D WENO5 (5 conserved variables, ghost 3), SSPRK3                        */
+/*   K3 = code/snapshot/warp_weno5_3d_rk3/kernels.py                         */
+/*   S3 = code/snapshot/warp_weno5_3d_rk3/solver.py                          */
+/*   MN3 = code/snapshot/warp_weno5_3d_rk3/mlp_normal_scalar.py              */
+/*   MK3 = code/snapshot/warp_weno5_3d_rk3/mlp_kernels.py                    */
+/* The reference fixes gamma = 1.4 and evaluates every expression unfused.   */
+/* ======================================================================== */
+typedef struct { double v[5]; } vec5;
+
+#define IDX3(k, j, i, c, nyt, nxt) \
+    (((((size_t)(k)) * (size_t)(nyt) + (size_t)(j)) * (size_t)(nxt) + (size_t)(i)) * 5u + (size_t)(c))
+
+static vec5 load5(const double *a, int k, int j, int i, int nyt, int nxt) {
+    vec5 r;
+    memcpy(r.v, a + IDX3(k, j, i, 0, nyt, nxt), sizeof r.v);
+    return r;
+}
+static void store5(double *a, int k, int j, int i, int nyt, int nxt, vec5 v) {
+    memcpy(a + IDX3(k, j, i, 0, nyt, nxt), v.v, sizeof v.v);
+}
+
+/* K3:29-56 */
+static vec5 con_to_pri5(vec5 c) {
+    vec5 p;
+    p.v[0] = c.v[0];
+    p.v[1] = c.v[1] / c.v[0];
+    p.v[2] = c.v[2] / c.v[0];
+    p.v[3] = c.v[3] / c.v[0];
+    p.v[4] = (1.4 - 1.0) * (c.v[4] - 0.5 * c.v[0] * (p.v[1] * p.v[1] + p.v[2] * p.v[2] + p.v[3] * p.v[3]));
+    return p;
+}
+static vec5 pri_to_con5(vec5 p) {
+    vec5 c;
+    c.v[0] = p.v[0];
+    c.v[1] = p.v[0] * p.v[1];
+    c.v[2] = p.v[0] * p.v[2];
+    c.v[3] = p.v[0] * p.v[3];
+    c.v[4] = p.v[4] / (1.4 - 1.0) + 0.5 * p.v[0] * (p.v[1] * p.v[1] + p.v[2] * p.v[2] + p.v[3] * p.v[3]);
+    return c;
+}
+/* K3:58-97 */
+static vec5 flux5(vec5 p, int dir) {
+    double rho = p.v[0], u = p.v[1], v = p.v[2], w = p.v[3], pr = p.v[4];
+    double e = 0.5 * rho * (u * u + v * v + w * w) + pr / (1.4 - 1.0);
+    vec5 f;
+    if (dir == 1) { f.v[0] = rho * u; f.v[1] = rho * u * u + pr; f.v[2] = rho * u * v; f.v[3] = rho * u * w; f.v[4] = u * (pr + e); }
+    else if (dir == 2) { f.v[0] = rho * v; f.v[1] = rho * u * v; f.v[2] = rho * v * v + pr; f.v[3] = rho * v * w; f.v[4] = v * (pr + e); }
+    else { f.v[0] = rho * w; f.v[1] = rho * u * w; f.v[2] = rho * v * w; f.v[3] = rho * w * w + pr; f.v[4] = w * (pr + e); }
+    return f;
+}
+static vec5 clamped_flux5(vec5 p, int dir) {
+    p.v[0] = dmax(p.v[0], 1.0e-15);
+    p.v[4] = dmax(p.v[4], 1.0e-15);
+    return flux5(p, dir);
+}
+
+/* K3:122-219 evilin_state */
+static vec5 evilin5(vec5 ul0, vec5 ur0, int dir, double dt_dx) {
+    const double gamma = 1.4, eps = 1.0e-15;
+    vec5 wl0 = con_to_pri5(ul0), wr0 = con_to_pri5(ur0);
+    wl0.v[0] = dmax(wl0.v[0], eps); wl0.v[4] = dmax(wl0.v[4], eps);
+    wr0.v[0] = dmax(wr0.v[0], eps); wr0.v[4] = dmax(wr0.v[4], eps);
+    double unl0 = wl0.v[dir], unr0 = wr0.v[dir];
+    double al0 = sqrt(gamma * wl0.v[4] / wl0.v[0]), ar0 = sqrt(gamma * wr0.v[4] / wr0.v[0]);
+    double smax = dmax(dmax(fabs(unl0) + al0, fabs(unr0) + ar0), eps);
+    double dx_dt = 1.0 / dt_dx, courant = smax * dt_dx;
+    vec5 fl = clamped_flux5(wl0, dir), fr = clamped_flux5(wr0, dir), u_lw, f_fo, f_gf, ulh, urh;
+    for (int c = 0; c < 5; ++c) u_lw.v[c] = 0.5 * (ul0.v[c] + ur0.v[c]) - 0.5 * dt_dx * (fr.v[c] - fl.v[c]);
+    vec5 w_lw = con_to_pri5(u_lw);
+    w_lw.v[0] = dmax(w_lw.v[0], eps); w_lw.v[4] = dmax(w_lw.v[4], eps);
+    vec5 f_lw = clamped_flux5(w_lw, dir);
+    for (int c = 0; c < 5; ++c) f_fo.v[c] = 0.25 * (fl.v[c] + 2.0 * f_lw.v[c] + fr.v[c] - dx_dt * (ur0.v[c] - ul0.v[c]));
+    double theta = (1.0 - courant) / (1.0 + courant);
+    for (int c = 0; c < 5; ++c) f_gf.v[c] = theta * f_lw.v[c] + (1.0 - theta) * f_fo.v[c];
+    for (int c = 0; c < 5; ++c) {
+        ulh.v[c] = ul0.v[c] - dt_dx * (f_gf.v[c] - fl.v[c]);
+        urh.v[c] = ur0.v[c] - dt_dx * (fr.v[c] - f_gf.v[c]);
+    }
+    vec5 wl = con_to_pri5(ulh), wr = con_to_pri5(urh);
+    wl.v[0] = dmax(wl.v[0], eps); wl.v[4] = dmax(wl.v[4], eps);
+    wr.v[0] = dmax(wr.v[0], eps); wr.v[4] = dmax(wr.v[4], eps);
+    double rho_bar = dmax(0.5 * (wl.v[0] + wr.v[0]), eps), p_bar = dmax(0.5 * (wl.v[4] + wr.v[4]), eps);
+    double a_bar = dmax(sqrt(gamma * p_bar / rho_bar), eps);
+    double c1 = rho_bar * a_bar, c2 = rho_bar / a_bar;
+    double unle = wl.v[dir], unre = wr.v[dir];
+    double u_star = 0.5 * (unle + unre) - 0.5 * (wr.v[4] - wl.v[4]) / c1;
+    double p_star = 0.5 * (wl.v[4] + wr.v[4]) - 0.5 * (unre - unle) * c1;
+    double rho_star_l = dmax(wl.v[0] + (unle - u_star) * c2, eps);
+    double rho_star_r = dmax(wr.v[0] + (u_star - unre) * c2, eps);
+    p_star = dmax(p_star, eps);
+    double u_bar = 0.5 * (unle + unre);
+    vec5 wf = wr;
+    if (u_bar - a_bar >= 0.0) wf = wl;
+    else if (u_bar >= 0.0) { wf = wl; wf.v[0] = rho_star_l; wf.v[dir] = u_star; wf.v[4] = p_star; }
+    else if (u_bar + a_bar >= 0.0) { wf = wr; wf.v[0] = rho_star_r; wf.v[dir] = u_star; wf.v[4] = p_star; }
+    wf.v[0] = dmax(wf.v[0], eps);
+    wf.v[4] = dmax(wf.v[4], eps);
+    return pri_to_con5(wf);
+}
+
+/* HLLC for five variables (no reference counterpart in 3-D): the AJS:23-115
+ * formula with both tangential velocities carried through the star state */
+static vec5 hllc5(vec5 ul0, vec5 ur0, int dir) {
+    const double gamma = 1.4, tiny = 1.0e-16;
+    vec5 wl = con_to_pri5(ul0), wr = con_to_pri5(ur0);
+    double rho_l = dmax(wl.v[0], tiny), rho_r = dmax(wr.v[0], tiny);
+    double p_l = dmax(wl.v[4], tiny), p_r = dmax(wr.v[4], tiny);
+    double a_l = sqrt(gamma * p_l / rho_l), a_r = sqrt(gamma * p_r / rho_r);
+    double un_l = wl.v[dir], un_r = wr.v[dir];
+    double s_l = dmin(un_l - a_l, un_r - a_r), s_r = dmax(un_l + a_l, un_r + a_r);
+    vec5 f_l = flux5(wl, dir), f_r = flux5(wr, dir);
+    if (0.0 <= s_l) return f_l;
+    if (0.0 >= s_r) return f_r;
+    double denom = floor_signed(rho_l * (s_l - un_l) - rho_r * (s_r - un_r), tiny);
+    double s_star = (p_r - p_l + rho_l * un_l * (s_l - un_l) - rho_r * un_r * (s_r - un_r)) / denom;
+    vec5 u_side = ul0, w_side = wl, f_side = f_l;
+    double s_side = s_l;
+    if (s_star < 0.0) { u_side = ur0; w_side = wr; f_side = f_r; s_side = s_r; }
+    double rho = dmax(w_side.v[0], tiny), p = dmax(w_side.v[4], tiny), un = w_side.v[dir];
+    double factor = rho * (s_side - un) / floor_signed(s_side - s_star, tiny);
+    vec5 us;
+    us.v[0] = factor;
+    for (int c = 1; c <= 3; ++c) us.v[c] = factor * w_side.v[c];
+    us.v[dir] = factor * s_star;
+    us.v[4] = factor * ((u_side.v[4] / rho) + (s_star - un) * (s_star + p / (rho * floor_signed(s_side - un, tiny))));
+    vec5 out;
+    for (int c = 0; c < 5; ++c) out.v[c] = f_side.v[c] + s_side * (us.v[c] - u_side.v[c]);
+    return out;
+}
+
+/* K3:233-269 roe_average (no floors) */
+static vec5 roe5(vec5 l, vec5 r) {
+    const double gamma = 1.4;
+    double rho_l = l.v[0], u_l = l.v[1] / rho_l, v_l = l.v[2] / rho_l, w_l = l.v[3] / rho_l, e_l = l.v[4];
+    double p_l = (gamma - 1.0) * (e_l - 0.5 * rho_l * (u_l * u_l + v_l * v_l + w_l * w_l));
+    double h_l = (e_l + p_l) / rho_l;
+    double rho_r = r.v[0], u_r = r.v[1] / rho_r, v_r = r.v[2] / rho_r, w_r = r.v[3] / rho_r, e_r = r.v[4];
+    double p_r = (gamma - 1.0) * (e_r - 0.5 * rho_r * (u_r * u_r + v_r * v_r + w_r * w_r));
+    double h_r = (e_r + p_r) / rho_r;
+    double sl = sqrt(rho_l), sr = sqrt(rho_r), inv = 1.0 / (sl + sr);
+    double u = (sl * u_l + sr * u_r) * inv, v = (sl * v_l + sr * v_r) * inv, w = (sl * w_l + sr * w_r) * inv;
+    double h = (sl * h_l + sr * h_r) * inv, rho = sl * sr, q2 = u * u + v * v + w * w;
+    double e = (rho * h + 0.5 * (gamma - 1.0) * rho * q2) / gamma;
+    vec5 o = {{rho, rho * u, rho * v, rho * w, e}};
+    return o;
+}
+
+/* K3:272-498 eigenvectors of the x / y / z Jacobians at a conserved state */
+static void eigen5(vec5 s, int dir, double L[5][5], double R[5][5]) {
+    double rho = s.v[0], u = s.v[1] / rho, v = s.v[2] / rho, w = s.v[3] / rho, energy = s.v[4];
+    double gm1 = 1.4 - 1.0, vtot = u * u + v * v + w * w;
+    double pressure = gm1 * (energy - 0.5 * rho * vtot);
+    double sound = sqrt(1.4 * pressure / rho), sound2 = sound * sound;
+    double enthalpy = 0.5 * vtot + sound2 / gm1;
+    double nn = dir == 1 ? u : dir == 2 ? v : w;       /* normal velocity */
+    double t1 = dir == 1 ? v : u, t2 = dir == 3 ? v : w; /* the two tangential velocities */
+    int in = dir, i1 = dir == 1 ? 2 : 1, i2 = dir == 3 ? 2 : 3;
+    memset(L, 0, sizeof(double) * 25);
+    memset(R, 0, sizeof(double) * 25);
+    L[0][0] = enthalpy + sound * (nn - sound) / gm1; L[0][in] = -(nn + sound / gm1); L[0][i1] = -t1; L[0][i2] = -t2; L[0][4] = 1.0;
+    L[1][0] = -2.0 * enthalpy + 4.0 * sound2 / gm1; L[1][in] = 2.0 * nn; L[1][i1] = 2.0 * t1; L[1][i2] = 2.0 * t2; L[1][4] = -2.0;
+    L[2][0] = -2.0 * t1 * sound2 / gm1; L[2][i1] = 2.0 * sound2 / gm1;
+    L[3][0] = -2.0 * t2 * sound2 / gm1; L[3][i2] = 2.0 * sound2 / gm1;
+    L[4][0] = enthalpy - sound * (nn + sound) / gm1; L[4][in] = -nn + sound / gm1; L[4][i1] = -t1; L[4][i2] = -t2; L[4][4] = 1.0;
+    if (dir == 1) L[4][in] = -(nn - sound / gm1);
+    double scale = gm1 / (2.0 * sound2);
+    for (int a = 0; a < 5; ++a)
+        for (int b = 0; b < 5; ++b) L[a][b] = L[a][b] * scale;
+    R[0][0] = 1.0; R[0][1] = 1.0; R[0][4] = 1.0;
+    R[in][0] = nn - sound; R[in][1] = nn; R[in][4] = nn + sound;
+    R[i1][0] = t1; R[i1][1] = t1; R[i1][2] = 1.0; R[i1][4] = t1;
+    R[i2][0] = t2; R[i2][1] = t2; R[i2][3] = 1.0; R[i2][4] = t2;
+    R[4][0] = enthalpy - nn * sound; R[4][1] = 0.5 * vtot; R[4][2] = t1; R[4][3] = t2; R[4][4] = enthalpy + nn * sound;
+}
+
+static vec5 matvec5(double M[5][5], vec5 x) {
+    vec5 r;
+    for (int a = 0; a < 5; ++a) {
+        double t = 0.0;
+        for (int b = 0; b < 5; ++b) t = t + M[a][b] * x.v[b];
+        r.v[a] = t;
+    }
+    return r;
+}
+
+typedef struct {
+    int scheme, eno_cutoff, riemann;
+    const void *mlp;
+} closure3_t;
+
+/* K3:570-588 characteristic_weno_face (lr 2: Roe(q1, q2); lr 1: Roe(q2, q3)) */
+static vec5 char_face5(const vec5 q[5], int lr, int dir, const closure3_t *cl) {
+    vec5 avg = lr == 1 ? roe5(q[2], q[3]) : roe5(q[1], q[2]);
+    double L[5][5], R[5][5];
+    eigen5(avg, dir, L, R);
+    vec5 c[5], f;
+    for (int m = 0; m < 5; ++m) c[m] = matvec5(L, q[m]);
+    int head = lr == 1 ? HEAD_NORMAL_LR1 : HEAD_NORMAL_LR2;
+    for (int k = 0; k < 5; ++k) {
+        double s[5] = {c[0].v[k], c[1].v[k], c[2].v[k], c[3].v[k], c[4].v[k]};
+        f.v[k] = scalar_value(s, head, cl->scheme, cl->mlp, cl->eno_cutoff, 0.0);
+    }
+    return matvec5(R, f);
+}
+
+/* K3:872-942 normal kernels: LR=2 first, then LR=1 on the stencil
+ * round-tripped through the LR=2 eigenvectors (the C++ in-place mutation) */
+static void normal3(const double *q, double *left, double *right, int nx, int ny, int nz, int dir,
+                    const closure3_t *cl) {
+    const int nxt = nx + 6, nyt = ny + 6, nzt = nz + 6;
+    int kn = dir == 3 ? nz + 2 : nzt, jn = dir == 2 ? ny + 2 : nyt, in = dir == 1 ? nx + 2 : nxt;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (int k = 0; k < kn; ++k)
+        for (int j = 0; j < jn; ++j)
+            for (int i = 0; i < in; ++i) {
+                vec5 s[5], r[5];
+                for (int m = 0; m < 5; ++m)
+                    s[m] = load5(q, k + (dir == 3 ? m : 0), j + (dir == 2 ? m : 0), i + (dir == 1 ? m : 0), nyt, nxt);
+                store5(left, k, j, i, nyt, nxt, char_face5(s, 2, dir, cl));
+                vec5 avg = roe5(s[1], s[2]);
+                double L[5][5], R[5][5];
+                eigen5(avg, dir, L, R);
+                for (int m = 0; m < 5; ++m) r[m] = matvec5(R, matvec5(L, s[m]));
+                store5(right, k, j, i, nyt, nxt, char_face5(r, 1, dir, cl));
+            }
+}
+
+/* K3:945-1017 transverse Gauss pass along tdir over the index box [0,kn)x[0,jn)x[0,in);
+ * the stencil starts one cell after the target index */
+static void transverse3(const double *src, double *dst, int nx, int ny, int kn, int jn, int in, int tdir, int lr,
+                        const closure3_t *cl) {
+    const int nxt = nx + 6, nyt = ny + 6;
+    int head = lr == 1 ? HEAD_GAUSS_LR1 : HEAD_GAUSS_LR2;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+    for (int k = 0; k < kn; ++k)
+        for (int j = 0; j < jn; ++j)
+            for (int i = 0; i < in; ++i) {
+                vec5 out;
+                for (int c = 0; c < 5; ++c) {
+                    double s[5];
+                    for (int m = 0; m < 5; ++m)
+                        s[m] = src[IDX3(k + (tdir == 3 ? m + 1 : 0), j + (tdir == 2 ? m + 1 : 0),
+                                        i + (tdir == 1 ? m + 1 : 0), c, nyt, nxt)];
+                    out.v[c] = scalar_value(s, head, cl->scheme, cl->mlp, cl->eno_cutoff, 0.0);
+                }
+                store5(dst, k, j, i, nyt, nxt, out);
+            }
+}
+
+/* S3:46-164 the four face points of one direction (flag order 1..4) and the
+ * flux sum F = sum_flag flux * dt/h (K3:1020-1090: EVILIN with the states in
+ * (right, left) argument order, i.e. physically (left side, right side)) */
+static void fluxes3(const double *q, double *flux, double *wk, int nx, int ny, int nz, int dir, double dt_h,
+                    const closure3_t *cl) {
+    const int nxt = nx + 6, nyt = ny + 6, nzt = nz + 6;
+    const size_t np = (size_t)nxt * nyt * nzt * 5;
+    double *nl = wk, *nr = wk + np, *sl = wk + 2 * np, *sr = wk + 3 * np, *pl = wk + 4 * np, *pr = wk + 5 * np;
+    normal3(q, nl, nr, nx, ny, nz, dir, cl);
+    /* first / second transverse directions and the index boxes of the passes */
+    int t1 = dir == 1 ? 2 : dir == 2 ? 1 : 2, t2 = dir == 3 ? 1 : 3;
+    int k1 = dir == 3 ? nz + 2 : nzt, j1 = dir == 2 ? ny + 2 : ny, i1 = dir == 1 ? nx + 2 : (dir == 2 ? nx : nxt);
+    if (dir == 3) { j1 = ny; i1 = nxt; }
+    int k2 = dir == 3 ? nz + 2 : nz, j2 = dir == 2 ? ny + 2 : ny, i2 = dir == 1 ? nx + 2 : nx;
+    int fk = dir == 3 ? nz + 1 : nz, fj = dir == 2 ? ny + 1 : ny, fi = dir == 1 ? nx + 1 : nx;
+    for (int flag = 1; flag <= 4; ++flag) {
+        int first_lr = (flag == 1 || flag == 3) ? 1 : 2;
+        int second_lr = (flag == 1 || flag == 2) ? 1 : 2;
+        transverse3(nl, sl, nx, ny, k1, j1, i1, t1, first_lr, cl);
+        transverse3(nr, sr, nx, ny, k1, j1, i1, t1, first_lr, cl);
+        transverse3(sl, pl, nx, ny, k2, j2, i2, t2, second_lr, cl);
+        transverse3(sr, pr, nx, ny, k2, j2, i2, t2, second_lr, cl);
+#pragma omp parallel for collapse(2) schedule(static)
+        for (int k = 0; k < fk; ++k)
+            for (int j = 0; j < fj; ++j)
+                for (int i = 0; i < fi; ++i) {
+                    vec5 left_state = load5(pl, k + (dir == 3), j + (dir == 2), i + (dir == 1), nyt, nxt);
+                    vec5 right_state = load5(pr, k, j, i, nyt, nxt);
+                    vec5 value = cl->riemann == RIEMANN7_HLLC
+                                     ? hllc5(right_state, left_state, dir)
+                                     : flux5(con_to_pri5(evilin5(right_state, left_state, dir, dt_h)), dir);
+                    for (int c = 0; c < 5; ++c) {
+                        size_t o = ((((size_t)k * fj + j) * fi + i) * 5 + c);
+                        double scaled = value.v[c] * dt_h;
+                        flux[o] = flag == 1 ? scaled : flux[o] + scaled;
+                    }
+                }
+    }
+}
+
+/* K3:811-853 transmissive ghost fill, in place, x then y then z */
+static void boundary3(double *q, int nx, int ny, int nz) {
+    const int nxt = nx + 6, nyt = ny + 6, nzt = nz + 6;
+    for (int k = 0; k < nzt; ++k)
+        for (int j = 0; j < nyt; ++j)
+            for (int c = 0; c < 5; ++c) {
+                double a = q[IDX3(k, j, 3, c, nyt, nxt)], b = q[IDX3(k, j, nx + 2, c, nyt, nxt)];
+                for (int g = 0; g < 3; ++g) { q[IDX3(k, j, g, c, nyt, nxt)] = a; q[IDX3(k, j, nx + 3 + g, c, nyt, nxt)] = b; }
+            }
+    for (int k = 0; k < nzt; ++k)
+        for (int i = 0; i < nxt; ++i)
+            for (int c = 0; c < 5; ++c) {
+                double a = q[IDX3(k, 3, i, c, nyt, nxt)], b = q[IDX3(k, ny + 2, i, c, nyt, nxt)];
+                for (int g = 0; g < 3; ++g) { q[IDX3(k, g, i, c, nyt, nxt)] = a; q[IDX3(k, ny + 3 + g, i, c, nyt, nxt)] = b; }
+            }
+    for (int j = 0; j < nyt; ++j)
+        for (int i = 0; i < nxt; ++i)
+            for (int c = 0; c < 5; ++c) {
+                double a = q[IDX3(3, j, i, c, nyt, nxt)], b = q[IDX3(nz + 2, j, i, c, nyt, nxt)];
+                for (int g = 0; g < 3; ++g) { q[IDX3(g, j, i, c, nyt, nxt)] = a; q[IDX3(nz + 3 + g, j, i, c, nyt, nxt)] = b; }
+            }
+}
+
+size_t oracle_euler3_work_doubles(int nx, int ny, int nz) {
+    size_t np = (size_t)(nx + 6) * (ny + 6) * (nz + 6) * 5;
+    size_t nf = (size_t)(nz + 1) * (ny + 1) * (nx + 1) * 5;
+    return 7 * np + 3 * nf;
+}
+
+/* S3:166-210 one SSPRK3 step of q (padded, in place; ghost cells refilled each stage) */
+int oracle_euler3_step(double *q, int nx, int ny, int nz, double dx, double dy, double dz, double dt, int scheme,
+                       int riemann, int eno_cutoff, const void *mlp) {
+    if ((scheme == SCHEME_SR64 || scheme == SCHEME_SR32 || scheme == SCHEME_SRBF16) && !mlp) return -2;
+    const int nxt = nx + 6, nyt = ny + 6;
+    const size_t np = (size_t)(nx + 6) * (ny + 6) * (nz + 6) * 5;
+    double *work = (double *)malloc(sizeof(double) * oracle_euler3_work_doubles(nx, ny, nz));
+    if (!work) return -3;
+    double *q0 = work, *wk = work + np;
+    double *fx = work + 7 * np, *fy = fx + (size_t)nz * ny * (nx + 1) * 5, *fz = fy + (size_t)nz * (ny + 1) * nx * 5;
+    closure3_t cl = {scheme, eno_cutoff, riemann, mlp};
+    for (int stage = 1; stage <= 3; ++stage) {
+        boundary3(q, nx, ny, nz);
+        fluxes3(q, fx, wk, nx, ny, nz, 1, dt / dx, &cl);
+        fluxes3(q, fy, wk, nx, ny, nz, 2, dt / dy, &cl);
+        fluxes3(q, fz, wk, nx, ny, nz, 3, dt / dz, &cl);
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j)
+                for (int i = 0; i < nx; ++i)
+                    for (int c = 0; c < 5; ++c) {
+                        size_t e = IDX3(k + 3, j + 3, i + 3, c, nyt, nxt);
+                        double fxl = fx[(((size_t)k * ny + j) * (nx + 1) + i) * 5 + c];
+                        double fxr = fx[(((size_t)k * ny + j) * (nx + 1) + i + 1) * 5 + c];
+                        double fyl = fy[(((size_t)k * (ny + 1) + j) * nx + i) * 5 + c];
+                        double fyr = fy[(((size_t)k * (ny + 1) + j + 1) * nx + i) * 5 + c];
+                        double fzl = fz[(((size_t)k * ny + j) * nx + i) * 5 + c];
+                        double fzr = fz[(((size_t)(k + 1) * ny + j) * nx + i) * 5 + c];
+                        double old = q[e];
+                        double adv = old - 0.25 * (fxr - fxl) - 0.25 * (fyr - fyl) - 0.25 * (fzr - fzl);
+                        if (stage == 1) { q0[e] = old; q[e] = adv; }
+                        else if (stage == 2) q[e] = (3.0 / 4.0) * q0[e] + (1.0 / 4.0) * adv;
+                        else q[e] = (1.0 / 3.0) * q0[e] + (2.0 / 3.0) * adv;
+                    }
+    }
+    free(work);
+    return 0;
+}
+
+/* K3:855-867 max speed over the interior primitive state (no floors) */
+double oracle_euler3_max_speed(const double *q, int nx, int ny, int nz) {
+    const int nxt = nx + 6, nyt = ny + 6;
+    double m = 0.0;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                vec5 p = con_to_pri5(load5(q, k + 3, j + 3, i + 3, nyt, nxt));
+                double a = sqrt(1.4 * p.v[4] / p.v[0]);
+                double s = dmax(fabs(p.v[1]) + a, dmax(fabs(p.v[2]) + a, fabs(p.v[3]) + a));
+                m = (s > m || s != s) ? s : m;
+            }
+    return m;
+}
+
+/* S3:184-210 advance: dt = cfl * min(dx, dy, dz) / max speed (1e10 below 1e-15), clipped to t_end */
+int oracle_euler3_run(double *q, int nx, int ny, int nz, double dx, double dy, double dz, double cfl, double t_end,
+                      int max_steps, int scheme, int riemann, int eno_cutoff, const void *mlp, double *dts,
+                      int *steps_out, double *t_out) {
+    double t = 0.0;
+    int step = 0;
+    while (step < max_steps && t != t_end) {
+        double m = oracle_euler3_max_speed(q, nx, ny, nz);
+        double dt = m < 1.0e-15 ? 1.0e10 : cfl * dmin(dx, dmin(dy, dz)) / m;
+        if (t + dt > t_end) dt = t_end - t;
+        int rc = oracle_euler3_step(q, nx, ny, nz, dx, dy, dz, dt, scheme, riemann, eno_cutoff, mlp);
+        if (rc) return rc;
+        if (dts) dts[step] = dt;
+        t += dt;
+        step += 1;
+    }
+    *steps_out = step;
+    *t_out = t;
+    return 0;
+}
+
+void oracle_flux3(const double *ul, const double *ur, int dir, double dt_h, int riemann, double *out) {
+    vec5 a, b;
+    memcpy(a.v, ul, sizeof a.v);
+    memcpy(b.v, ur, sizeof b.v);
+    vec5 f = riemann == RIEMANN7_HLLC ? hllc5(a, b, dir) : flux5(con_to_pri5(evilin5(a, b, dir, dt_h)), dir);
+    memcpy(out, f.v, sizeof f.v);
+}
+void oracle_eigen3(const double *state, int dir, double *L, double *R) {
+    vec5 s;
+    memcpy(s.v, state, sizeof s.v);
+    double l[5][5], r[5][5];
+    eigen5(s, dir, l, r);
+    memcpy(L, l, sizeof l);
+    memcpy(R, r, sizeof r);
+}

@@ -258,3 +258,95 @@ def make_double_mach_state(nx: int = 1200, ny: int = 300, ghost: int = 4, quadra
             y = yc + 0.5 * dy * float(ay)
             state += float(wx) * float(wy) * double_mach_conserved(x, y, 0.0, gamma)
     return 0.25 * state
+
+
+# ---------------------------------------------------------------- 3-D shock-bubble
+_GAUSS3_NODES = (-0.77459666924148340427791481488384306430816650390625, 0.0,
+                 0.77459666924148340427791481488384306430816650390625)
+_GAUSS3_WEIGHTS = (0.55555555555555546920487586248782463371753692626953125,
+                   0.88888888888888917261255073754000477492809295654296875,
+                   0.55555555555555546920487586248782463371753692626953125)
+
+
+def shockbubble_ma3_conserved(x, y, z, gamma: float = 1.4) -> np.ndarray:
+    """code/snapshot/warp_weno5_3d_rk3/kernels.py:730-767: Mach-3 shock in air at
+    x < 0.005 and a helium sphere (radius 0.025 at (0.035, 0.0445, 0.0445))."""
+    mach, rho_air, p_air, rho_he = 3.0, 1.29, 101325.0, 0.214
+    rho_post = (((gamma + 1.0) * mach * mach) / ((gamma - 1.0) * mach * mach + 2.0)) * rho_air
+    p_post = ((((2.0 * gamma) * mach * mach) - (gamma - 1.0)) / (gamma + 1.0)) * p_air
+    x, y, z = np.broadcast_arrays(np.asarray(x, float), np.asarray(y, float), np.asarray(z, float))
+    rho = np.full(x.shape, rho_air)
+    u = np.zeros(x.shape)
+    p = np.full(x.shape, p_air)
+    shocked = x < 0.005
+    rho = np.where(shocked, rho_post, rho)
+    u = np.where(shocked, 736.911, u)
+    p = np.where(shocked, p_post, p)
+    dxb, dyb, dzb = x - 0.035, y - 0.0445, z - 0.0445
+    bubble = dxb * dxb + dyb * dyb + dzb * dzb <= 0.025 * 0.025
+    rho = np.where(bubble, rho_he, rho)
+    u = np.where(bubble, 0.0, u)
+    p = np.where(bubble, p_air, p)
+    out = np.empty(x.shape + (5,))
+    out[..., 0] = rho
+    out[..., 1] = rho * u
+    out[..., 2] = 0.0
+    out[..., 3] = 0.0
+    out[..., 4] = 0.5 * rho * (u * u) + p / (gamma - 1.0)
+    return out
+
+
+def make_shockbubble_ma3_state(nx: int = 224, ny: int = 88, nz: int = 88, ghost: int = 3,
+                               lengths=(0.225, 0.089, 0.089)) -> np.ndarray:
+    """kernels.py:770-799 initialize_shockbubble_kernel: 3 x 3 x 3 Gauss-Legendre
+    cell averages over the padded grid (sum over x, then y, then z nodes)."""
+    dx, dy, dz = lengths[0] / nx, lengths[1] / ny, lengths[2] / nz
+    kk, jj, ii = np.indices((nz + 2 * ghost, ny + 2 * ghost, nx + 2 * ghost))
+    xc = (ii - ghost + 0.5) * dx
+    yc = (jj - ghost + 0.5) * dy
+    zc = (kk - ghost + 0.5) * dz
+    total = np.zeros(xc.shape + (5,))
+    for gi in range(3):
+        x = xc + 0.5 * dx * _GAUSS3_NODES[gi]
+        for gj in range(3):
+            y = yc + 0.5 * dy * _GAUSS3_NODES[gj]
+            for gk in range(3):
+                z = zc + 0.5 * dz * _GAUSS3_NODES[gk]
+                w = _GAUSS3_WEIGHTS[gi] * _GAUSS3_WEIGHTS[gj] * _GAUSS3_WEIGHTS[gk]
+                total = total + w * shockbubble_ma3_conserved(x, y, z)
+    return 0.125 * total
+
+
+def make_shockbubble_bench_state(nx: int = 224, ny: int = 88, nz: int = 88, ghost: int = 3, mach: float = 3.0,
+                                 density_ratio: float = 0.138, gamma: float = 1.4) -> np.ndarray:
+    """A planar Mach-3 shock in a uniform gas (rho = 1, p = 1 ahead of it) striking
+    a spherical light-gas bubble of radius ny/4 cells with density ratio 0.138
+    in pressure equilibrium, on a unit-spacing grid (cell averages by 3 x 3 x 3
+    Gauss-Legendre quadrature).  No reference counterpart: the benchmark case."""
+    rho_post = ((gamma + 1.0) * mach * mach) / ((gamma - 1.0) * mach * mach + 2.0)
+    p_post = (2.0 * gamma * mach * mach - (gamma - 1.0)) / (gamma + 1.0)
+    c_pre = np.sqrt(gamma)
+    u_post = mach * c_pre * (1.0 - 1.0 / rho_post)
+    radius = 0.25 * ny
+    centre = (0.5 * ny + 8.0, 0.5 * ny, 0.5 * nz)
+    kk, jj, ii = np.indices((nz + 2 * ghost, ny + 2 * ghost, nx + 2 * ghost))
+    xc, yc, zc = ii - ghost + 0.5, jj - ghost + 0.5, kk - ghost + 0.5
+    total = np.zeros(xc.shape + (5,))
+    for gi in range(3):
+        x = xc + 0.5 * _GAUSS3_NODES[gi]
+        for gj in range(3):
+            y = yc + 0.5 * _GAUSS3_NODES[gj]
+            for gk in range(3):
+                z = zc + 0.5 * _GAUSS3_NODES[gk]
+                w = _GAUSS3_WEIGHTS[gi] * _GAUSS3_WEIGHTS[gj] * _GAUSS3_WEIGHTS[gk]
+                shocked = x < 4.0
+                bubble = (x - centre[0]) ** 2 + (y - centre[1]) ** 2 + (z - centre[2]) ** 2 <= radius * radius
+                rho = np.where(shocked, rho_post, np.where(bubble, density_ratio, 1.0))
+                u = np.where(shocked, u_post, 0.0)
+                p = np.where(shocked, p_post, 1.0)
+                q = np.zeros(xc.shape + (5,))
+                q[..., 0] = rho
+                q[..., 1] = rho * u
+                q[..., 4] = 0.5 * rho * u * u + p / (gamma - 1.0)
+                total = total + w * q
+    return 0.125 * total
