@@ -33,8 +33,9 @@ enum {
 enum {
     WSR_RIEMANN_HLLC = 0,  /* code/snapshot/for_paper_results/solvers/weno5_hllc.py:23-115 */
     WSR_RIEMANN_FORCE = 1, /* code/snapshot/warp_weno5_helpers.py:401-435 */
-    WSR_RIEMANN_EVILIN = 2 /* WENO7 only: code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/
-                              warp_weno7_ader4_helpers_classical_only.py:256-342,429-433 */
+    WSR_RIEMANN_EVILIN = 2 /* WENO7 2-D: code/snapshot/teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/
+                              warp_weno7_ader4_helpers_classical_only.py:256-342,429-433;
+                              3-D WENO5: code/snapshot/warp_weno5_3d_rk3/kernels.py:122-219 */
 };
 /* boundary conditions */
 enum {
@@ -221,6 +222,48 @@ int wsr_euler7_step(const wsr_euler7_config *cfg, double *u, double dt, double t
 int wsr_euler7_loop_init(const wsr_euler7_config *cfg, const double *u, int32_t max_steps, void *workspace,
                          size_t workspace_bytes, void *stream);
 int wsr_euler7_loop_steps(const wsr_euler7_config *cfg, double *u, double *dts, int32_t n_steps, void *workspace,
+                          size_t workspace_bytes, void *stream);
+
+/* ------------------------------------------------------------------ 3-D WENO5
+ * The 3-D characteristic WENO5 step of code/snapshot/warp_weno5_3d_rk3:
+ * state (nz+6, ny+6, nx+6, 5) FP64 conserved (rho, rho u, rho v, rho w, E),
+ * ghost 3; per direction a characteristic normal reconstruction (5 x 5
+ * eigenvectors, kernels.py:272-498, 870-942), two transverse Gauss passes
+ * giving the four face points (kernels.py:941-1014, solver.py:165-283), the
+ * flux at each point (EVILIN, kernels.py:122-219, 1016-1090, or the
+ * five-variable HLLC, which the reference does not have in 3-D) and the
+ * SSPRK3 update (kernels.py:1092-1163).  Schemes: JS, SR64, SR32, SRBF16
+ * (no WENO-Z in the reference's 3-D path).  Boundary: transmissive. */
+typedef struct wsr_euler3_config {
+    int32_t nx, ny, nz, ghost;
+    int32_t scheme, riemann, boundary;
+    int32_t eno_cutoff;
+    int32_t flags;
+    int32_t reserved;
+    double dx, dy, dz, gamma;
+    double cfl, t_end;
+    const void *mlp;
+} wsr_euler3_config;
+
+size_t wsr_euler3_workspace_bytes(const wsr_euler3_config *cfg);
+
+/* One SSPRK3 step of q in place with a given dt (solver.py:285-301 _rk_stage
+ * x 3 as called by advance, solver.py:303-329). */
+int wsr_euler3_step(const wsr_euler3_config *cfg, double *q, double dt, void *workspace, size_t workspace_bytes,
+                    void *stream);
+
+/* Max over the interior of max(|u|+a, |v|+a, |w|+a) (solver.py:159-163,
+ * kernels.py:855-867), NaN if any cell's speed is NaN. */
+int wsr_euler3_max_speed(const wsr_euler3_config *cfg, const double *q, void *workspace, size_t workspace_bytes,
+                         double *max_speed_out, void *stream);
+
+/* Device-resident time loop replacing solver.py:303-329 advance and the
+ * run_solver loop (solver.py:348-...): dt = cfl min(dx,dy,dz) / max speed
+ * (1e10 below 1e-15), clipped to t_end; stops at t == t_end or max_steps.
+ * wsr_euler2d_loop_status reads the same control block. */
+int wsr_euler3_loop_init(const wsr_euler3_config *cfg, const double *q, int32_t max_steps, void *workspace,
+                         size_t workspace_bytes, void *stream);
+int wsr_euler3_loop_steps(const wsr_euler3_config *cfg, double *q, double *dts, int32_t n_steps, void *workspace,
                           size_t workspace_bytes, void *stream);
 
 const char *wsr_version(void);

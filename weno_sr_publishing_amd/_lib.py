@@ -23,6 +23,8 @@ EXPORTS = (
     "wsr_adv1d_train_forward", "wsr_adv1d_train_adjoint",
     "wsr_euler7_workspace_bytes", "wsr_euler7_rhs", "wsr_euler7_step", "wsr_euler7_loop_init",
     "wsr_euler7_loop_steps",
+    "wsr_euler3_workspace_bytes", "wsr_euler3_step", "wsr_euler3_max_speed", "wsr_euler3_loop_init",
+    "wsr_euler3_loop_steps",
 )
 FLAG_COUNT_CLOSURE = 1
 FLAG_Z_FALLBACK = 2
@@ -49,6 +51,18 @@ class Euler7Config(ctypes.Structure):
         ("eno_cutoff", ctypes.c_int32), ("flags", ctypes.c_int32),
         ("loop_mode", ctypes.c_int32), ("reserved", ctypes.c_int32),
         ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("gamma", ctypes.c_double),
+        ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
+        ("mlp", ctypes.c_void_p),
+    ]
+
+
+class Euler3Config(ctypes.Structure):
+    """Mirror of ``wsr_euler3_config``."""
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("ghost", ctypes.c_int32),
+        ("scheme", ctypes.c_int32), ("riemann", ctypes.c_int32), ("boundary", ctypes.c_int32),
+        ("eno_cutoff", ctypes.c_int32), ("flags", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double), ("gamma", ctypes.c_double),
         ("cfl", ctypes.c_double), ("t_end", ctypes.c_double),
         ("mlp", ctypes.c_void_p),
     ]
@@ -120,6 +134,17 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler7_loop_init.argtypes = [C7, P, I32, P, SZ, P]
         L.wsr_euler7_loop_steps.restype = ctypes.c_int
         L.wsr_euler7_loop_steps.argtypes = [C7, P, P, I32, P, SZ, P]
+        C3 = ctypes.POINTER(Euler3Config)
+        L.wsr_euler3_workspace_bytes.restype = SZ
+        L.wsr_euler3_workspace_bytes.argtypes = [C3]
+        L.wsr_euler3_step.restype = ctypes.c_int
+        L.wsr_euler3_step.argtypes = [C3, P, D, P, SZ, P]
+        L.wsr_euler3_max_speed.restype = ctypes.c_int
+        L.wsr_euler3_max_speed.argtypes = [C3, P, P, SZ, ctypes.POINTER(D), P]
+        L.wsr_euler3_loop_init.restype = ctypes.c_int
+        L.wsr_euler3_loop_init.argtypes = [C3, P, I32, P, SZ, P]
+        L.wsr_euler3_loop_steps.restype = ctypes.c_int
+        L.wsr_euler3_loop_steps.argtypes = [C3, P, P, I32, P, SZ, P]
         L.wsr_version.restype = ctypes.c_char_p
         _lib = L
     return _lib

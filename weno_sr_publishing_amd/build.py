@@ -14,7 +14,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libwsr_hip.so"
-SOURCES = [CSRC / "wsr_euler2d.hip", CSRC / "wsr_euler7.hip", CSRC / "wsr_adv1d.hip", CSRC / "wsr_train1d.hip"]
+SOURCES = [CSRC / "wsr_euler2d.hip", CSRC / "wsr_euler7.hip", CSRC / "wsr_euler3.hip", CSRC / "wsr_adv1d.hip", CSRC / "wsr_train1d.hip"]
 HEADERS = [CSRC / "wsr_math.hpp", CSRC / "wsr_loop.hpp", CSRC / "wsr_weno7.hpp", CSRC / "wsr_adv1d.hpp", CSRC / "wsr_adjoint.hpp", PKG.parent / "include" / "wsr.h"]
 ARCH = os.environ.get("WSR_OFFLOAD_ARCH", "gfx950")
 

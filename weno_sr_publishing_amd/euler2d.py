@@ -53,7 +53,10 @@ class Euler2DSolver:
         self.riemann = riemann
         self.boundary = boundary
         self.eno_cutoff = bool(eno_cutoff)
-        self.device = torch.device(device)
+        dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self._mlp_host = None
         if scheme in ("sr64", "sr32", "srbf16"):
             if mlp is None:
