@@ -68,12 +68,219 @@ def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -
     return nfaces * (ntrans + 4) * (FACE_LINE_FIXED + 8 * n) + nfaces * ntrans * (FLUX_FACE_FIXED + 8 * g)
 
 
+def cpu_model_name() -> str:
+    name = platform.processor() or ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return name
+
+
+# ------------------------------------------------------------- 3-D workload
+# 224 x 88 x 88 Mach-3 shock striking a light-gas bubble (rho = 1, p = 1 ahead of
+# the shock, bubble radius 0.25 ny cells, density ratio 0.138), unit cell size,
+# HLLC, SSPRK3, CFL 0.4, 200 fixed steps.
+NX3, NY3, NZ3, STEPS3 = 224, 88, 88, 200
+# algorithmic operations of the 3-D passes (same counting rules as above)
+ROE5, EIG5 = 45, 22               # Roe average of two 5-variable states; eigen-system scalars
+ROW5, DOT5, BACK5 = 15, 9, 10     # one scaled left row; one 5-term projection; one back-projection column
+ROUNDTRIP5 = 5 * (5 * (ROW5 + DOT5) + 5 * BACK5)   # R (L q) of the five stencil cells
+HLLC5 = 125                        # five-variable HLLC flux
+FLUX_FACE3 = 4 * HLLC5 + 4 * 5 + 3 * 5             # four Gauss-point fluxes, dt/h scaling, flag sum
+
+
+def gauss_cost(scheme: str, frac: float) -> float:
+    if scheme.startswith("sr"):
+        return GAUSS["sr"][0] + frac * (GAUSS["sr"][1] - GAUSS["sr"][0])
+    return GAUSS[scheme]
+
+
+def normal_cost(scheme: str, frac: float) -> float:
+    if scheme.startswith("sr"):
+        return NORMAL["sr"][0] + frac * (NORMAL["sr"][1] - NORMAL["sr"][0])
+    return NORMAL[scheme]
+
+
+def flux3_flops_per_launch(scheme: str, frac_g: float) -> float:
+    """k3_flux, averaged over the x / y / z launches: per face 20 Gauss-pair
+    reconstructions (2 first-pass variants x 2 sides x 5 components) + 4 fluxes."""
+    nf = ((NX3 + 1) * NY3 * NZ3 + NX3 * (NY3 + 1) * NZ3 + NX3 * NY3 * (NZ3 + 1)) / 3.0
+    return nf * (20 * gauss_cost(scheme, frac_g) + FLUX_FACE3)
+
+
+def trans13_flops_per_launch(scheme: str, frac_g: float) -> float:
+    """k3_trans1: 10 Gauss-pair reconstructions (2 sides x 5 components) per position."""
+    npos = ((NX3 + 2) * NY3 * (NZ3 + 4) + NX3 * (NY3 + 2) * (NZ3 + 4) + (NX3 + 4) * NY3 * (NZ3 + 2)) / 3.0
+    return npos * 10 * gauss_cost(scheme, frac_g)
+
+
+def normal3_flops_per_launch(scheme: str, frac_n: float) -> float:
+    """k3_normal: LR = 2 and LR = 1 (on R (L q)) characteristic reconstructions per position."""
+    npos = ((NX3 + 2) * (NY3 + 4) * (NZ3 + 4) + (NX3 + 4) * (NY3 + 2) * (NZ3 + 4)
+            + (NX3 + 4) * (NY3 + 4) * (NZ3 + 2)) / 3.0
+    per_field = ROW5 + 5 * DOT5 + normal_cost(scheme, frac_n) + BACK5
+    return npos * (2 * (ROE5 + EIG5 + 5 * per_field) + ROUNDTRIP5 + ROE5 + EIG5)
+
+
+def bench_shockbubble3d(args) -> int:
+    import torch
+    from weno_sr_publishing_amd.checkpoints import load_f32, load_f64
+    from weno_sr_publishing_amd.euler3d import Euler3Solver, ShockBubbleConfig, primitive3
+    from weno_sr_publishing_amd.initial import make_shockbubble_bench_state
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}), flush=True)
+        return 1
+    dev = torch.device("cuda:0")
+    steps = args.steps if args.steps > 0 else STEPS3
+    q_host = make_shockbubble_bench_state(NX3, NY3, NZ3)
+    cfg = ShockBubbleConfig(nx=NX3, ny=NY3, nz=NZ3, x_end=float(NX3), y_end=float(NY3), z_end=float(NZ3),
+                            cfl=0.4, t_end=float("inf"))
+    mlp32 = load_f32()
+    mlps = {"sr32": mlp32, "sr64": load_f64(), "srbf16": mlp32, "js": None}
+    ncell = NX3 * NY3 * NZ3
+
+    def make(scheme):
+        return Euler3Solver(cfg, scheme=scheme, riemann="hllc", mlp=mlps[scheme], device=dev)
+
+    def timed(scheme, instrument=False):
+        s = make(scheme)
+        q_init = s.new_state(q_host)
+        q = torch.empty_like(q_init)
+        if args.warmup > 0:
+            q.copy_(q_init)
+            s.run(q, float("inf"), max_steps=args.warmup, chunk=args.warmup)
+        q.copy_(q_init)
+        torch.cuda.synchronize()
+        ms, nl = [0.0, 0.0, 0.0], [0, 0, 0]
+        t0 = time.perf_counter()
+        s.loop_init(q, float("inf"), steps)
+        while True:
+            if instrument:
+                kms, kn = s.loop_steps_timed(q, 25)
+                ms = [a + b for a, b in zip(ms, kms)]
+                nl = [a + b for a, b in zip(nl, kn)]
+            else:
+                s.loop_steps(q, 50)
+            n, t, _, done = s.loop_status()
+            if done or n >= steps:
+                break
+        torch.cuda.synchronize()
+        return s, q, n, time.perf_counter() - t0, ms, nl
+
+    s_sr, q_sr, n_sr, wall_sr, _, _ = timed(args.closure)
+    _, q_js, n_js, wall_js, _, _ = timed("js")
+    pri = primitive3(q_sr.cpu().numpy()[3:-3, 3:-3, 3:-3])
+    healthy = bool(np.isfinite(pri).all() and pri[..., 0].min() > 0 and pri[..., 4].min() > 0)
+    _, _, _, wall_i, kms, kn = timed(args.closure, instrument=True)
+    _, _, _, wall_js_i, kms_js, kn_js = timed("js", instrument=True)
+
+    s_cnt = make(args.closure)
+    s_cnt.set_counting(True)
+    q_cnt = s_cnt.new_state(q_host)
+    s_cnt.run(q_cnt, float("inf"), max_steps=steps, chunk=50)
+    c = s_cnt.counters()
+    frac_n, frac_g = c[1] / max(1, c[0]), c[3] / max(1, c[2])
+    flux_ms = kms[2] / max(1, kn[2])
+    flops = flux3_flops_per_launch(args.closure, frac_g)
+    achieved = flops / (flux_ms * 1e-3) / 1e12
+    js_flux_ms = kms_js[2] / max(1, kn_js[2])
+    js_achieved = flux3_flops_per_launch("js", 0.0) / (js_flux_ms * 1e-3) / 1e12
+    per_kernel = {}
+    for k, name in enumerate(("k3_normal", "k3_trans1", "k3_flux")):
+        avg = kms[k] / max(1, kn[k])
+        fl = (normal3_flops_per_launch(args.closure, frac_n), trans13_flops_per_launch(args.closure, frac_g),
+              flops)[k]
+        per_kernel[name] = {"avg_launch_ms": avg, "launches": kn[k], "algorithmic_flops_per_launch": fl,
+                            "achieved_TFLOPs": fl / (avg * 1e-3) / 1e12}
+    # algorithmic HBM bytes of one flux launch: 20 first-pass stencil values read per lane pair x 5 ... counted
+    # as the unique data: four first-pass planes (2 x 5 x 8 B per position) + the flux planes (5 x 8 B per face)
+    nf = ((NX3 + 1) * NY3 * NZ3 + NX3 * (NY3 + 1) * NZ3 + NX3 * NY3 * (NZ3 + 1)) / 3.0
+    flux_bytes = nf * (4 * 5 * 8 + 5 * 8)
+    value = ncell * n_sr / wall_sr
+    out = {
+        "metric": "3D Euler cell-updates/s, WENO5-SR vs WENO-JS, 224x88x88 Mach-3 shock-bubble",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": n_sr,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_sr / n_sr,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: planar Mach-3 shock (rho=1, p=1 ahead) and a light-gas bubble (radius 0.25 ny, "
+                "density ratio 0.138), 3x3x3 Gauss-Legendre cell averages; trained WENO5-SR checkpoints",
+        "config": {"workload": f"3D Euler 224x88x88 shock-bubble, WENO5-SR ({args.closure} closure) + HLLC + "
+                               f"SSPRK3, CFL 0.4, {n_sr} fixed steps", "nx": NX3, "ny": NY3, "nz": NZ3,
+                   "closure": args.closure, "state": "fp64", "riemann": "hllc", "boundary": "transmissive"},
+        "wall_s": wall_sr,
+        "healthy": healthy,
+        "wall_s_event_instrumented": {"sr": wall_i, "js": wall_js_i},
+        "js": {"value": ncell * n_js / wall_js, "steps": n_js, "wall_s": wall_js,
+               "ms_per_step": 1e3 * wall_js / n_js},
+        "sr_vs_js_wall": wall_sr / wall_js,
+        "roofline": {
+            "bound": "mfma",
+            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
+                       "FP64-vector / non-packed FP32-vector rate",
+            "kernel": f"k3_flux<x|y|z,{args.closure},hllc>",
+            "achieved": achieved,
+            "peak": PEAK_VALU_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_VALU_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": flux_ms,
+            "launches": kn[2],
+            "algorithmic_flops_per_launch": flops,
+            "closure_fraction": {"normal": frac_n, "gauss": frac_g},
+            "closure_wave_fraction": {"normal": c[4] / max(1, c[0]), "gauss": c[5] / max(1, c[2])},
+            "hbm_algorithmic_GBs": flux_bytes / (flux_ms * 1e-3) / 1e9,
+            "hbm_frac": flux_bytes / (flux_ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+            "per_kernel": per_kernel,
+            "js_flux": {"avg_launch_ms": js_flux_ms, "achieved": js_achieved,
+                        "frac": js_achieved / PEAK_VALU_TFLOPS},
+        },
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        s_mid = make(args.closure)
+        q_mid = s_mid.new_state(q_host)
+        s_mid.run(q_mid, float("inf"), max_steps=max(1, n_sr // 2), chunk=50)
+        qm = q_mid.cpu().numpy()
+        done_steps, t0 = 0, time.perf_counter()
+        while True:
+            qm, _, k, _ = oracle.run3d(qm, NX3, NY3, NZ3, 1.0, 1.0, 1.0, 0.4, float("inf"), max_steps=1,
+                                       scheme=args.closure, riemann="hllc", mlp=mlps[args.closure])
+            done_steps += k
+            el = time.perf_counter() - t0
+            if k == 0 or el >= args.cpu_seconds or done_steps >= 20:
+                break
+        cpu_rate = ncell * done_steps / el
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{done_steps} SSPRK3 steps of the 224x88x88 case ({args.closure} closure, "
+                                         f"HLLC) from the step-{max(1, n_sr // 2)} state, OpenMP C oracle, {el:.1f} s",
+                               "cpu_model": cpu_model_name(), "host_cpus": os.cpu_count()}
+        out["gpu_vs_cpu"] = value / cpu_rate
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed SSPRK3 steps (0 = full trajectory to t=0.25)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--closure", default="sr32", choices=["sr32", "sr64", "srbf16"])
+    ap.add_argument("--workload", default="c4-400", choices=["c4-400", "shockbubble3d"],
+                    help="c4-400: the headline 2-D line; shockbubble3d: the 3-D 224x88x88 Mach-3 "
+                         "shock-bubble leg (200 fixed steps), printed as its own JSON line")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
@@ -91,6 +298,9 @@ def main() -> int:
                                    "one MI355X (multi-GPU domain decomposition is design-only)",
                           "n_gpus": args.gpus}), flush=True)
         return 2
+
+    if args.workload == "shockbubble3d":
+        return bench_shockbubble3d(args)
 
     import torch
     from weno_sr_publishing_amd.checkpoints import load_f32, load_f64

@@ -266,6 +266,17 @@ int wsr_euler3_loop_init(const wsr_euler3_config *cfg, const double *q, int32_t 
 int wsr_euler3_loop_steps(const wsr_euler3_config *cfg, double *q, double *dts, int32_t n_steps, void *workspace,
                           size_t workspace_bytes, void *stream);
 
+/* Instrumented loop_steps (no reference counterpart; used by the benchmark):
+ * the same n_steps (<= 32) steps with a HIP event pair around every normal,
+ * first-transverse-pass and flux launch on `stream`; synchronises and returns
+ * the summed time (ms) and the number of launches that did work for each of
+ * the three kernel kinds (kernel_ms[3], launches[3]).  With
+ * WSR_FLAG_COUNT_CLOSURE in cfg->flags the closure counters of
+ * wsr_euler2d_counters accumulate over the 3-D passes as well. */
+int wsr_euler3_loop_steps_timed(const wsr_euler3_config *cfg, double *q, double *dts, int32_t n_steps,
+                                void *workspace, size_t workspace_bytes, void *stream, double *kernel_ms,
+                                int32_t *launches);
+
 const char *wsr_version(void);
 
 #ifdef __cplusplus

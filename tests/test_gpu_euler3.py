@@ -136,3 +136,33 @@ def test_shockbubble_ma3_sr64_matches_recorded_run(mlp64, gpu_device):
     assert d["nan_count"] == 0
     for key in ("rho_min", "rho_max", "p_min", "p_max"):
         assert d[key] == pytest.approx(rec[key], rel=1e-10), key
+
+
+@pytest.mark.parametrize("scheme", ["js", "sr32"])
+def test_closure_counters3(mlp64, mlp32, gpu_device, scheme):
+    """Counting closure evaluations changes no result, and the reconstruction
+    totals are those of the pass boxes: per stage and direction 10 normal
+    reconstructions per normal-box position, 10 Gauss pairs per first-pass
+    position and 20 per face."""
+    nx, ny, nz = 12, 10, 9
+    q = state3(nx, ny, nz, 4)
+    runs = []
+    for counting in (False, True):
+        s = make_solver(nx, ny, nz, scheme, "hllc", mlp64, mlp32)
+        s.set_counting(counting)
+        qd = s.new_state(q)
+        s.run(qd, 1.0, max_steps=2, chunk=2)
+        runs.append((qd.cpu().numpy(), s.counters() if counting else None))
+    assert np.array_equal(runs[0][0], runs[1][0])
+    c = runs[1][1]
+    n = {1: nx, 2: ny, 3: nz}
+    normal = ((nx + 2) * (ny + 4) * (nz + 4) + (nx + 4) * (ny + 2) * (nz + 4) + (nx + 4) * (ny + 4) * (nz + 2))
+    trans = ((nx + 2) * ny * (nz + 4) + nx * (ny + 2) * (nz + 4) + (nx + 4) * ny * (nz + 2))
+    faces = ((nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1))
+    stages = 2 * 3
+    assert c[0] == stages * 10 * normal
+    assert c[2] == stages * (10 * trans + 20 * faces)
+    if scheme == "js":
+        assert c[1] == 0 and c[3] == 0
+    else:
+        assert 0 < c[1] <= c[0] and 0 < c[3] <= c[2]

@@ -24,7 +24,7 @@ EXPORTS = (
     "wsr_euler7_workspace_bytes", "wsr_euler7_rhs", "wsr_euler7_step", "wsr_euler7_loop_init",
     "wsr_euler7_loop_steps",
     "wsr_euler3_workspace_bytes", "wsr_euler3_step", "wsr_euler3_max_speed", "wsr_euler3_loop_init",
-    "wsr_euler3_loop_steps",
+    "wsr_euler3_loop_steps", "wsr_euler3_loop_steps_timed",
 )
 FLAG_COUNT_CLOSURE = 1
 FLAG_Z_FALLBACK = 2
@@ -145,6 +145,8 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler3_loop_init.argtypes = [C3, P, I32, P, SZ, P]
         L.wsr_euler3_loop_steps.restype = ctypes.c_int
         L.wsr_euler3_loop_steps.argtypes = [C3, P, P, I32, P, SZ, P]
+        L.wsr_euler3_loop_steps_timed.restype = ctypes.c_int
+        L.wsr_euler3_loop_steps_timed.argtypes = [C3, P, P, I32, P, SZ, P, ctypes.POINTER(D), ctypes.POINTER(I32)]
         L.wsr_version.restype = ctypes.c_char_p
         _lib = L
     return _lib

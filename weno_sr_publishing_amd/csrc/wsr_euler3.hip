@@ -290,6 +290,7 @@ struct PassArgs {
     typename MlpSweep<SCHEME>::type mlp;
     int eno;
     const Ctl *ctl;
+    unsigned long long *counters;   // nullptr unless closure evaluations are counted
 };
 
 // kernels.py:802-852 boundary_{x,y,z}_kernel: transmissive copy, in place
@@ -321,13 +322,16 @@ __global__ __launch_bounds__(BLOCK) void k3_normal(const double *__restrict__ q,
     const int ex = DIR == 1 ? g.nx + 2 : g.nx + 4, ey = DIR == 2 ? g.ny + 2 : g.ny + 4;
     const int ez = DIR == 3 ? g.nz + 2 : g.nz + 4;
     const int ox = DIR == 1 ? 0 : 1, oy = DIR == 2 ? 0 : 1, oz = DIR == 3 ? 0 : 1;
+    const long long nbox = (long long)ex * ey * ez;
     long long idx = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    if (idx >= (long long)ex * ey * ez) return;
+    const bool active = idx < nbox;
+    if (!active) idx = nbox - 1;            // idle lanes recompute the last position, store nothing
     const int i = ox + (int)(idx % ex);
     long long r = idx / ex;
     const int j = oy + (int)(r % ey), k = oz + (int)(r / ey);
     const long long base = pidx(g, k, j, i), st = axis_stride<DIR>(g);
     const bool lr1 = blockIdx.y == 1;
+    LaneCounts cnt;
 
     V5 s[5];
 #pragma unroll
@@ -374,13 +378,17 @@ __global__ __launch_bounds__(BLOCK) void k3_normal(const double *__restrict__ q,
         for (int m = 0; m < 5; ++m) w[m] = dot5(l, s[m]);
         bool ev;
         double f = recon_normal<SCHEME>(w, lr1, A.mlp, A.eno, h, ev);
+        if (A.counters) cnt.add(0, 4, active, ev);
         right_col5<DIR>(kk, E, rc);
 #pragma unroll
         for (int a = 0; a < 5; ++a) out.v[a] = out.v[a] + rc[a] * f;
     }
     double *dst = lr1 ? NR : NL;
+    if (active) {
 #pragma unroll
-    for (int c = 0; c < 5; ++c) dst[c * g.np + base] = out.v[c];
+        for (int c = 0; c < 5; ++c) dst[c * g.np + base] = out.v[c];
+    }
+    if (A.counters) cnt.flush(A.counters);
 }
 
 // first transverse Gauss pass (kernels.py:941-1014): lane -> (side, comp, position)
@@ -399,7 +407,8 @@ __global__ __launch_bounds__(BLOCK) void k3_trans1(const double *__restrict__ NL
     const long long nbox = (long long)ex * ey * ez;
     long long idx = (long long)blockIdx.x * BLOCK + threadIdx.x;
     const int sc = blockIdx.y;                 // side * 5 + comp
-    if (idx >= nbox) return;
+    const bool active = idx < nbox;
+    if (!active) idx = nbox - 1;
     const int side = sc / 5, comp = sc - side * 5;
     const int i = ox + (int)(idx % ex);
     long long r = idx / ex;
@@ -414,8 +423,15 @@ __global__ __launch_bounds__(BLOCK) void k3_trans1(const double *__restrict__ NL
     bool ev;
     recon_gauss<SCHEME>(w, A.mlp, A.eno, h, g1, g2, ev);
     const long long plane = 5 * g.np;
-    S[(0 * 2 + side) * plane + comp * g.np + pos] = g1;
-    S[(1 * 2 + side) * plane + comp * g.np + pos] = g2;
+    if (active) {
+        S[(0 * 2 + side) * plane + comp * g.np + pos] = g1;
+        S[(1 * 2 + side) * plane + comp * g.np + pos] = g2;
+    }
+    if (A.counters) {
+        LaneCounts cnt;
+        cnt.add(2, 5, active, ev);
+        cnt.flush(A.counters);
+    }
 }
 
 // second transverse pass + Riemann solves + flag sum (solver.py:176-203,
@@ -447,6 +463,7 @@ __global__ __launch_bounds__(BLOCK) void k3_flux(const double *__restrict__ S, d
     const double *lo_src = S + (a * 2 + 1) * plane + pos;
     const double *hi_src = S + (a * 2 + 0) * plane + pos + sn;
     V5 lo1 = {}, lo2 = {}, hi1 = {}, hi2 = {};
+    LaneCounts cnt;
 #pragma unroll 1
     for (int c = 0; c < 5; ++c) {
         double w[5], g1, g2, e1, e2;
@@ -454,9 +471,11 @@ __global__ __launch_bounds__(BLOCK) void k3_flux(const double *__restrict__ S, d
 #pragma unroll
         for (int m = 0; m < 5; ++m) w[m] = lo_src[c * g.np + (m + 1) * st];
         recon_gauss<SCHEME>(w, A.mlp, A.eno, ht, g1, g2, ev);
+        if (A.counters) cnt.add(2, 5, active, ev);
 #pragma unroll
         for (int m = 0; m < 5; ++m) w[m] = hi_src[c * g.np + (m + 1) * st];
         recon_gauss<SCHEME>(w, A.mlp, A.eno, ht, e1, e2, ev);
+        if (A.counters) cnt.add(2, 5, active, ev);
 #pragma unroll
         for (int cc = 0; cc < 5; ++cc) {
             if (cc == c) { lo1.v[cc] = g1; lo2.v[cc] = g2; hi1.v[cc] = e1; hi2.v[cc] = e2; }
@@ -482,6 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k3_flux(const double *__restrict__ S, d
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[c * g.np + pos] = tot[c];
     }
+    if (A.counters) cnt.flush(A.counters);
 }
 
 // kernels.py:1092-1163 rk_stage{1,2,3}: q <- {adv, 3/4 q0 + 1/4 adv, 1/3 q0 + 2/3 adv},
@@ -674,54 +694,83 @@ static void enqueue_params3(const wsr_euler3_config *cfg, const Workspace3 &w, h
 
 static inline unsigned blocks_for(long long n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
 
+// optional HIP-event pairs around the normal / first-pass / flux launches
+constexpr int TIMER3_MAX_STEPS = 32;
+constexpr int TIMER3_PAIRS = TIMER3_MAX_STEPS * 3 * 3 * 3;
+struct Timer3 {
+    hipEvent_t ev[2 * TIMER3_PAIRS];
+    int kind[TIMER3_PAIRS];
+    int used = 0, cap = 0;
+    void begin(hipStream_t st) {
+        if (used < cap) (void)hipEventRecord(ev[2 * used], st);
+    }
+    void end(int k, hipStream_t st) {
+        if (used < cap) {
+            (void)hipEventRecord(ev[2 * used + 1], st);
+            kind[used] = k;
+            used += 1;
+        }
+    }
+};
+
 template <int DIR, int SCHEME, int RIEMANN>
 static void launch_direction(double *q, const Workspace3 &w, double *F, const Grid3 &g, int eno, const Ctl *ctl,
-                             double dt, hipStream_t st) {
+                             double dt, unsigned long long *counters, hipStream_t st, Timer3 *tm) {
     PassArgs<SCHEME> A;
     memset(&A, 0, sizeof A);
-    A.g = g; A.eno = eno; A.ctl = ctl;
+    A.g = g; A.eno = eno; A.ctl = ctl; A.counters = counters;
     if constexpr (SCHEME == SCHEME_SR64) A.mlp.p = (const double *)w.mlp;
     else if constexpr (SCHEME != SCHEME_JS) A.mlp.p = (const float *)w.mlp;
     const long long nnorm = (long long)(DIR == 1 ? g.nx + 2 : g.nx + 4) * (DIR == 2 ? g.ny + 2 : g.ny + 4) *
                             (DIR == 3 ? g.nz + 2 : g.nz + 4);
+    if (tm) tm->begin(st);
     hipLaunchKernelGGL((k3_normal<DIR, SCHEME>), dim3(blocks_for(nnorm), 2), dim3(BLOCK), 0, st, (const double *)q,
                        w.NL, w.NR, A);
+    if (tm) tm->end(0, st);
     constexpr int T1 = Trans<DIR>::T1;
     const long long ntr = (long long)(DIR == 1 ? g.nx + 2 : (T1 == 1 ? g.nx : g.nx + 4)) *
                           (DIR == 2 ? g.ny + 2 : g.ny) * (DIR == 3 ? g.nz + 2 : g.nz + 4);
+    if (tm) tm->begin(st);
     hipLaunchKernelGGL((k3_trans1<DIR, SCHEME>), dim3(blocks_for(ntr), 10), dim3(BLOCK), 0, st,
                        (const double *)w.NL, (const double *)w.NR, w.S, A);
+    if (tm) tm->end(1, st);
     const long long nf = (long long)(DIR == 1 ? g.nx + 1 : g.nx) * (DIR == 2 ? g.ny + 1 : g.ny) *
                          (DIR == 3 ? g.nz + 1 : g.nz);
+    if (tm) tm->begin(st);
     hipLaunchKernelGGL((k3_flux<DIR, SCHEME, RIEMANN>), dim3(blocks_for(2 * nf)), dim3(BLOCK), 0, st,
                        (const double *)w.S, F, A, dt);
+    if (tm) tm->end(2, st);
 }
 
 template <int SCHEME, int RIEMANN>
 static void enqueue_fluxes_t(double *q, const Workspace3 &w, const Grid3 &g, int eno, const Ctl *ctl, double dt,
-                             hipStream_t st) {
-    launch_direction<1, SCHEME, RIEMANN>(q, w, w.Fx, g, eno, ctl, dt, st);
-    launch_direction<2, SCHEME, RIEMANN>(q, w, w.Fy, g, eno, ctl, dt, st);
-    launch_direction<3, SCHEME, RIEMANN>(q, w, w.Fz, g, eno, ctl, dt, st);
+                             unsigned long long *counters, hipStream_t st, Timer3 *tm) {
+    launch_direction<1, SCHEME, RIEMANN>(q, w, w.Fx, g, eno, ctl, dt, counters, st, tm);
+    launch_direction<2, SCHEME, RIEMANN>(q, w, w.Fy, g, eno, ctl, dt, counters, st, tm);
+    launch_direction<3, SCHEME, RIEMANN>(q, w, w.Fz, g, eno, ctl, dt, counters, st, tm);
 }
 
 template <int SCHEME>
 static void enqueue_fluxes_s(int riemann, double *q, const Workspace3 &w, const Grid3 &g, int eno, const Ctl *ctl,
-                             double dt, hipStream_t st) {
-    if (riemann == WSR_RIEMANN_EVILIN) enqueue_fluxes_t<SCHEME, WSR_RIEMANN_EVILIN>(q, w, g, eno, ctl, dt, st);
-    else enqueue_fluxes_t<SCHEME, WSR_RIEMANN_HLLC>(q, w, g, eno, ctl, dt, st);
+                             double dt, unsigned long long *counters, hipStream_t st, Timer3 *tm) {
+    if (riemann == WSR_RIEMANN_EVILIN)
+        enqueue_fluxes_t<SCHEME, WSR_RIEMANN_EVILIN>(q, w, g, eno, ctl, dt, counters, st, tm);
+    else
+        enqueue_fluxes_t<SCHEME, WSR_RIEMANN_HLLC>(q, w, g, eno, ctl, dt, counters, st, tm);
 }
 
 // one stage: boundary fill, x / y / z fluxes, update (solver.py:285-301)
 static void enqueue_stage3(const wsr_euler3_config *cfg, const Workspace3 &w, double *q, int rk, double dt, Ctl *ctl,
-                           int fold_speed, double *dts, hipStream_t st) {
+                           int fold_speed, double *dts, hipStream_t st, Timer3 *tm = nullptr) {
     Grid3 g = make_grid3(cfg);
     hipLaunchKernelGGL(k3_boundary, dim3(blocks_for(g.np)), dim3(BLOCK), 0, st, q, g, (const Ctl *)ctl);
+    unsigned long long *cnt = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
+    const int eno = cfg->eno_cutoff, rs = cfg->riemann;
     switch (cfg->scheme) {
-    case WSR_SCHEME_JS: enqueue_fluxes_s<SCHEME_JS>(cfg->riemann, q, w, g, cfg->eno_cutoff, ctl, dt, st); break;
-    case WSR_SCHEME_SR64: enqueue_fluxes_s<SCHEME_SR64>(cfg->riemann, q, w, g, cfg->eno_cutoff, ctl, dt, st); break;
-    case WSR_SCHEME_SR32: enqueue_fluxes_s<SCHEME_SR32>(cfg->riemann, q, w, g, cfg->eno_cutoff, ctl, dt, st); break;
-    default: enqueue_fluxes_s<SCHEME_SRBF16>(cfg->riemann, q, w, g, cfg->eno_cutoff, ctl, dt, st); break;
+    case WSR_SCHEME_JS: enqueue_fluxes_s<SCHEME_JS>(rs, q, w, g, eno, ctl, dt, cnt, st, tm); break;
+    case WSR_SCHEME_SR64: enqueue_fluxes_s<SCHEME_SR64>(rs, q, w, g, eno, ctl, dt, cnt, st, tm); break;
+    case WSR_SCHEME_SR32: enqueue_fluxes_s<SCHEME_SR32>(rs, q, w, g, eno, ctl, dt, cnt, st, tm); break;
+    default: enqueue_fluxes_s<SCHEME_SRBF16>(rs, q, w, g, eno, ctl, dt, cnt, st, tm); break;
     }
     const long long ncells = (long long)g.nx * g.ny * g.nz;
     hipLaunchKernelGGL(k3_update, dim3(blocks_for(ncells)), dim3(BLOCK), 0, st, q, w.q0, (const double *)w.Fx,
@@ -811,6 +860,45 @@ int wsr_euler3_loop_steps(const wsr_euler3_config *cfg, double *q, double *dts, 
         enqueue_stage3(cfg, w, q, 3, 0.0, w.ctl, 1, dts, st);
     }
     return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler3_loop_steps_timed(const wsr_euler3_config *cfg, double *q, double *dts, int32_t n_steps,
+                                void *workspace, size_t workspace_bytes, void *stream, double *kernel_ms,
+                                int32_t *launches) {
+    int rc = check3(cfg);
+    if (rc) return rc;
+    if (!q || !dts || !workspace || n_steps < 0 || n_steps > TIMER3_MAX_STEPS || !kernel_ms || !launches)
+        return WSR_E_ARG;
+    if (workspace_bytes < ws3_layout(cfg, nullptr, nullptr)) return WSR_E_WORKSPACE;
+    Workspace3 w;
+    ws3_layout(cfg, &w, workspace);
+    hipStream_t st = (hipStream_t)stream;
+    static_assert(sizeof(Timer3) < (1u << 20), "timer record size");
+    Timer3 *tm = new Timer3();
+    tm->cap = 27 * n_steps;
+    for (int i = 0; i < 2 * tm->cap; ++i)
+        if (hipEventCreate(&tm->ev[i]) != hipSuccess) { delete tm; return WSR_E_HIP; }
+    if (n_steps > 0) enqueue_params3(cfg, w, st);
+    for (int n = 0; n < n_steps; ++n) {
+        hipLaunchKernelGGL(k3_dt, dim3(1), dim3(1), 0, st, w.ctl);
+        enqueue_stage3(cfg, w, q, 1, 0.0, w.ctl, 0, dts, st, tm);
+        enqueue_stage3(cfg, w, q, 2, 0.0, w.ctl, 0, dts, st, tm);
+        enqueue_stage3(cfg, w, q, 3, 0.0, w.ctl, 1, dts, st, tm);
+    }
+    rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) rc = WSR_E_HIP;
+    for (int k = 0; k < 3; ++k) { kernel_ms[k] = 0.0; launches[k] = 0; }
+    for (int i = 0; i < tm->used; ++i) {
+        float ms = 0.0f;
+        // launches after the loop stopped return at once; count only the ones that did work
+        if (hipEventElapsedTime(&ms, tm->ev[2 * i], tm->ev[2 * i + 1]) == hipSuccess && ms > 0.02f) {
+            kernel_ms[tm->kind[i]] += ms;
+            launches[tm->kind[i]] += 1;
+        }
+    }
+    for (int i = 0; i < 2 * tm->cap; ++i) (void)hipEventDestroy(tm->ev[i]);
+    delete tm;
+    return rc;
 }
 
 }  // extern "C"

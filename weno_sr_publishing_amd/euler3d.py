@@ -225,6 +225,32 @@ class Euler3Solver:
                                              self.workspace.data_ptr(), self.ws_bytes, self._stream())
         _lib.check(rc, "wsr_euler3_loop_steps")
 
+    def loop_steps_timed(self, q: torch.Tensor, n: int) -> tuple[list[float], list[int]]:
+        """n (<= 32) steps with HIP events around the normal / first-pass / flux
+        launches: ([ms] x 3, [launches] x 3) of the launches that did work."""
+        ms = (ctypes.c_double * 3)()
+        cnt = (ctypes.c_int32 * 3)()
+        rc = self._lib.wsr_euler3_loop_steps_timed(ctypes.byref(self.cfg), q.data_ptr(), self._dts.data_ptr(),
+                                                   int(n), self.workspace.data_ptr(), self.ws_bytes, self._stream(),
+                                                   ms, cnt)
+        _lib.check(rc, "wsr_euler3_loop_steps_timed")
+        return [float(v) for v in ms], [int(v) for v in cnt]
+
+    def set_counting(self, on: bool) -> None:
+        if on:
+            self.cfg.flags |= _lib.FLAG_COUNT_CLOSURE
+        else:
+            self.cfg.flags &= ~_lib.FLAG_COUNT_CLOSURE
+
+    def counters(self) -> list[int]:
+        """Closure counters since loop_init: [normal recons, of which closure ran,
+        Gauss-pair recons, of which closure ran, lanes of normal-closure waves,
+        lanes of Gauss-closure waves]."""
+        out = (ctypes.c_uint64 * 6)()
+        rc = self._lib.wsr_euler2d_counters(self.workspace.data_ptr(), out, 6, self._stream())
+        _lib.check(rc, "wsr_euler2d_counters")
+        return [int(v) for v in out]
+
     def loop_status(self) -> tuple[int, float, float, bool]:
         steps, t, dt, done = ctypes.c_int32(0), ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int32(0)
         rc = self._lib.wsr_euler2d_loop_status(self.workspace.data_ptr(), ctypes.byref(steps), ctypes.byref(t),
