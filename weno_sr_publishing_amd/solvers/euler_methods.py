@@ -1,5 +1,5 @@
-"""Method dispatch of code/snapshot/for_paper_results/solvers/euler_methods.py:27-78
-(WENO5 methods; WENO7 is not part of this library yet)."""
+"""Method dispatch of code/snapshot/for_paper_results/solvers/euler_methods.py:27-96
+(WENO5 methods on the 2-D WENO5 step, WENO7 methods on the WENO7 + Shu RK4 step)."""
 from __future__ import annotations
 
 from typing import Any
@@ -11,12 +11,34 @@ from ..params import Params
 from . import weno5_hllc, weno5_hllc_mixed, weno5_hllc_refsym
 
 WENO5_METHODS = ("weno5_js", "weno5_sr_f64", "weno5_sr_f32", "weno5_sr_bf16", "weno5_z")
+WENO7_METHODS = ("weno7_js", "weno7_sr_f64")
 
 
 def make_weno5_params(nx: int, ny: int, x_length: float, y_length: float, cfl: float, t_end: float,
                       mixed: bool = False) -> Params:
     del mixed  # the reference's mixed Params class is identical
     return Params(nx=nx, ny=ny, x_length=x_length, y_length=y_length, cfl=cfl, t_end=t_end)
+
+
+def make_weno7_params(nx: int, ny: int, x_min: float, x_max: float, y_min: float, y_max: float, cfl: float,
+                      t_end: float):
+    from ..euler7 import Params7
+    return Params7(nx=nx, ny=ny, x_min=x_min, x_max=x_max, y_min=y_min, y_max=y_max, cfl=cfl, t_end=t_end)
+
+
+def run_weno7(method_key: str, initial: np.ndarray, params, *, device: str, boundary: str,
+              report_interval: int = 0) -> tuple[np.ndarray, dict[str, Any]]:
+    """euler_methods.py:81-96: WENO7-JS or WENO7-SR (FP64 closure), HLLC, characteristic."""
+    from . import point_rk4, point_rk4_mlp
+    if method_key == "weno7_js":
+        return point_rk4.run_from_initial(initial, params, device=device, riemann_solver="hllc",
+                                          characteristic=True, boundary=boundary, report_interval=report_interval)
+    if method_key == "weno7_sr_f64":
+        beta = point_rk4_mlp.TorchWeno7PointBeta(checkpoints.WENO7_SR_FP64, device, params.gamma)
+        return point_rk4_mlp.run_from_initial_mlp(initial, params, device=device, riemann_solver="hllc",
+                                                  beta_model=beta, boundary=boundary, eno_cutoff=False,
+                                                  report_interval=report_interval)
+    raise ValueError(f"not a WENO7 paper method: {method_key}")
 
 
 def run_weno5(method_key: str, initial: np.ndarray, params: Params, *, device: str, boundary: str,
