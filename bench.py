@@ -68,6 +68,18 @@ def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -
     return nfaces * (ntrans + 4) * (FACE_LINE_FIXED + 8 * n) + nfaces * ntrans * (FLUX_FACE_FIXED + 8 * g)
 
 
+def copy_calibration(args, out: dict, torch, dev) -> None:
+    """One device copy of known size after the run: the reference kernel for the
+    hardware-counter byte calibration."""
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration_bytes"] = {"read": n * 8, "write": n * 8}
+
+
 def cpu_model_name() -> str:
     name = platform.processor() or ""
     try:
@@ -268,6 +280,176 @@ def bench_shockbubble3d(args) -> int:
                                          f"HLLC) from the step-{max(1, n_sr // 2)} state, OpenMP C oracle, {el:.1f} s",
                                "cpu_model": cpu_model_name(), "host_cpus": os.cpu_count()}
         out["gpu_vs_cpu"] = value / cpu_rate
+    copy_calibration(args, out, torch, dev)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+# ------------------------------------------------------------- WENO7 workload
+# Double Mach reflection on [0,4]x[0,1] at 1200x300 (Mach-10 shock at 60 degrees,
+# pre-shock rho=1.4, p=1), WENO7-SR with the FP32 closure and per-reconstruction
+# WENO7-Z fallback, HLLC, Shu RK4 with the downwind operator, CFL 0.4, t=0.2.
+NX7, NY7, T7 = 1200, 300, 0.2
+# algorithmic operations of one WENO7 scalar reconstruction (same counting rules)
+MLP7_RAW = 1952                           # 6-24-16-16-4: 1756 affine + 168 swish + 12 tanh head + 16 softmax
+SR7_CLOSURE = 2 * MLP7_RAW + 35 + 8 + 23  # + non-plateau features, reflection average, weights (beta = 4 r)
+FEAT7_ALWAYS = 50                         # four normalised deltas, plateau test
+NORMAL7 = {"js": 150, "z": 160, "sr": (28 + FEAT7_ALWAYS + 7, 28 + FEAT7_ALWAYS + 7 + SR7_CLOSURE)}
+GAUSS7 = {"js": 204, "z": 214, "sr": (56 + FEAT7_ALWAYS + 14, 56 + FEAT7_ALWAYS + 14 + SR7_CLOSURE)}
+FACE_LINE7_FIXED = 68 + 8 * (10 + 56 + 7)  # Roe + eigen; per lane: left row, 7 projections, back-projection
+
+
+def sweep7_flops_per_launch(scheme: str, frac_n: float, frac_g: float, both: bool) -> float:
+    """Mean of the x and y k7_sweep launches at 1200x300."""
+    if scheme.startswith("sr"):
+        n = NORMAL7["sr"][0] + frac_n * (NORMAL7["sr"][1] - NORMAL7["sr"][0])
+        g = GAUSS7["sr"][0] + frac_g * (GAUSS7["sr"][1] - GAUSS7["sr"][0])
+    else:
+        n, g = NORMAL7[scheme], GAUSS7[scheme]
+    flux = (4 if both else 2) * 110 + (8 if both else 4)
+    total = 0.0
+    for nfaces, ntrans in ((NX7 + 1, NY7), (NY7 + 1, NX7)):
+        total += nfaces * (ntrans + 6) * (FACE_LINE7_FIXED + 8 * n) + nfaces * ntrans * (8 * g + flux)
+    return total / 2.0
+
+
+def bench_doublemach7(args) -> int:
+    import torch
+    from weno_sr_publishing_amd.checkpoints import WENO7_SR_FP64, load_weno7_checkpoint
+    from weno_sr_publishing_amd.common import interior_stats
+    from weno_sr_publishing_amd.euler7 import Euler7Solver, double_mach_params7
+    from weno_sr_publishing_amd.initial import make_double_mach_state
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}), flush=True)
+        return 1
+    dev = torch.device("cuda:0")
+    params = double_mach_params7(NX7, NY7, cfl=0.4, t_end=T7)
+    u_host = make_double_mach_state(NX7, NY7)
+    mlp64 = load_weno7_checkpoint(WENO7_SR_FP64)
+    closure = {"sr32": "sr32", "sr64": "sr64", "srbf16": "sr32"}[args.closure]
+    mlps = {"sr32": mlp64.as_precision("fp32"), "sr64": mlp64, "js": None}
+    max_steps = args.steps if args.steps > 0 else 10_000_000
+
+    def make(scheme):
+        return Euler7Solver(params, scheme, "hllc", "double-mach", mlps[scheme], loop="double-mach",
+                            z_fallback=scheme != "js", device=dev)
+
+    def timed(scheme, instrument=False):
+        s = make(scheme)
+        u_init = s.new_state(u_host)
+        u = torch.empty_like(u_init)
+        if args.warmup > 0:
+            u.copy_(u_init)
+            s.run(u, T7, max_steps=args.warmup, chunk=args.warmup)
+        u.copy_(u_init)
+        torch.cuda.synchronize()
+        ms, nl = [0.0, 0.0], [0, 0]
+        t0 = time.perf_counter()
+        s.loop_init(u, T7, max_steps)
+        while True:
+            if instrument:
+                kms, kn = s.loop_steps_timed(u, 32)
+                ms = [a + b for a, b in zip(ms, kms)]
+                nl = [a + b for a, b in zip(nl, kn)]
+            else:
+                s.loop_steps(u, 64)
+            n, t, _, done = s.loop_status()
+            if done or n >= max_steps:
+                break
+        torch.cuda.synchronize()
+        return s, u, n, t, time.perf_counter() - t0, ms, nl
+
+    _, u_sr, n_sr, t_sr, wall_sr, _, _ = timed(closure)
+    _, _, n_js, t_js, wall_js, _, _ = timed("js")
+    stats = interior_stats(u_sr.cpu().numpy(), params)
+    _, _, _, _, wall_i, kms, kn = timed(closure, instrument=True)
+    _, _, _, _, wall_js_i, kms_js, kn_js = timed("js", instrument=True)
+    s_cnt = make(closure)
+    s_cnt.set_counting(True)
+    u_cnt = s_cnt.new_state(u_host)
+    s_cnt.run(u_cnt, T7, max_steps=max_steps)
+    c = s_cnt.counters()
+    frac_n, frac_g = c[1] / max(1, c[0]), c[3] / max(1, c[2])
+    both_ms = kms[0] / max(1, kn[0])
+    flops = sweep7_flops_per_launch(closure, frac_n, frac_g, True)
+    achieved = flops / (both_ms * 1e-3) / 1e12
+    js_both_ms = kms_js[0] / max(1, kn_js[0])
+    js_achieved = sweep7_flops_per_launch("js", 0.0, 0.0, True) / (js_both_ms * 1e-3) / 1e12
+    ncell = NX7 * NY7
+    value = ncell * n_sr / wall_sr
+    hbm = ((NX7 + 8) * (NY7 + 8) * 32 + ((NX7 + 1) * NY7 + NX7 * (NY7 + 1)) / 2.0 * 64) / (both_ms * 1e-3) / 1e9
+    out = {
+        "metric": "2D Euler cell-updates/s, WENO7-SR vs WENO7-JS, 1200x300 double Mach reflection",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": n_sr,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall_sr / n_sr,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: double Mach reflection initial state (15x15 Gauss-Legendre cell averages); trained "
+                "WENO7-SR checkpoint (FP64 parameters rounded to FP32 for the FP32 closure)",
+        "config": {"workload": f"2D Euler double Mach 1200x300, WENO7-SR ({closure} closure, WENO7-Z fallback) + "
+                               f"HLLC + Shu RK4, CFL 0.4, t=0..{T7 if args.steps <= 0 else round(t_sr, 6)}",
+                   "nx": NX7, "ny": NY7, "closure": closure, "state": "fp64", "riemann": "hllc",
+                   "boundary": "double-mach", "t_end": t_sr},
+        "wall_s": wall_sr,
+        "health": stats,
+        "wall_s_event_instrumented": {"sr": wall_i, "js": wall_js_i},
+        "js": {"value": ncell * n_js / wall_js, "steps": n_js, "wall_s": wall_js,
+               "ms_per_step": 1e3 * wall_js / n_js},
+        "sr_vs_js_wall": wall_sr / wall_js,
+        "roofline": {
+            "bound": "mfma",
+            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
+                       "FP64-vector / non-packed FP32-vector rate",
+            "kernel": f"k7_sweep<x|y,{closure},hllc,L+L~>",
+            "achieved": achieved,
+            "peak": PEAK_VALU_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_VALU_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": both_ms,
+            "launches": kn[0],
+            "l_only_sweep": {"avg_launch_ms": kms[1] / max(1, kn[1]), "launches": kn[1]},
+            "algorithmic_flops_per_launch": flops,
+            "closure_fraction": {"normal": frac_n, "gauss": frac_g},
+            "closure_wave_fraction": {"normal": c[4] / max(1, c[0]), "gauss": c[5] / max(1, c[2])},
+            "hbm_algorithmic_GBs": hbm,
+            "hbm_frac": hbm / PEAK_HBM_GBS,
+            "js_sweep": {"avg_launch_ms": js_both_ms, "achieved": js_achieved,
+                         "frac": js_achieved / PEAK_VALU_TFLOPS},
+        },
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        s_mid = make(closure)
+        u_mid = s_mid.new_state(u_host)
+        s_mid.run(u_mid, T7, max_steps=max(1, n_sr // 2))
+        um = u_mid.cpu().numpy()
+        done_steps, t0 = 0, time.perf_counter()
+        while True:
+            um, _, k, _ = oracle.run7_2d(um, NX7, NY7, params.dx, params.dy, params.gamma, params.cfl, T7,
+                                         scheme=closure, riemann="hllc", boundary="double-mach",
+                                         mlp=mlps[closure], mode="double-mach", max_steps=1)
+            done_steps += k
+            el = time.perf_counter() - t0
+            if k == 0 or el >= args.cpu_seconds or done_steps >= 50:
+                break
+        cpu_rate = ncell * done_steps / el
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{done_steps} Shu RK4 steps of the 1200x300 double Mach case ({closure} "
+                                         f"closure, no Z fallback, HLLC) from the step-{max(1, n_sr // 2)} state, "
+                                         f"OpenMP C oracle, {el:.1f} s",
+                               "cpu_model": cpu_model_name(), "host_cpus": os.cpu_count()}
+        out["gpu_vs_cpu"] = value / cpu_rate
+    copy_calibration(args, out, torch, dev)
     print(json.dumps(out), flush=True)
     return 0
 
@@ -278,9 +460,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=0, help="timed SSPRK3 steps (0 = full trajectory to t=0.25)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--closure", default="sr32", choices=["sr32", "sr64", "srbf16"])
-    ap.add_argument("--workload", default="c4-400", choices=["c4-400", "shockbubble3d"],
+    ap.add_argument("--workload", default="c4-400", choices=["c4-400", "shockbubble3d", "doublemach7"],
                     help="c4-400: the headline 2-D line; shockbubble3d: the 3-D 224x88x88 Mach-3 "
-                         "shock-bubble leg (200 fixed steps), printed as its own JSON line")
+                         "shock-bubble leg (200 fixed steps); doublemach7: WENO7 + Shu RK4 double Mach "
+                         "reflection at 1200x300 to t=0.2; each prints its own JSON line")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
@@ -301,6 +484,8 @@ def main() -> int:
 
     if args.workload == "shockbubble3d":
         return bench_shockbubble3d(args)
+    if args.workload == "doublemach7":
+        return bench_doublemach7(args)
 
     import torch
     from weno_sr_publishing_amd.checkpoints import load_f32, load_f64
@@ -483,13 +668,7 @@ def main() -> int:
                                "cpu_model": cpu_model, "host_cpus": os.cpu_count()}
         out["gpu_vs_cpu"] = value / cpu_rate
 
-    if args.copy_calibration > 0:
-        n = args.copy_calibration * (1 << 20) // 8
-        src = torch.ones(n, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        out["copy_calibration_bytes"] = {"read": n * 8, "write": n * 8}
+    copy_calibration(args, out, torch, dev)
     print(json.dumps(out), flush=True)
     return 0
 

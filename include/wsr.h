@@ -224,6 +224,15 @@ int wsr_euler7_loop_init(const wsr_euler7_config *cfg, const double *u, int32_t 
 int wsr_euler7_loop_steps(const wsr_euler7_config *cfg, double *u, double *dts, int32_t n_steps, void *workspace,
                           size_t workspace_bytes, void *stream);
 
+/* Instrumented loop_steps (no reference counterpart; used by the benchmark):
+ * n_steps (<= 32) steps with a HIP event pair around every sweep launch on
+ * `stream`; synchronises and returns the summed sweep time (ms) and launch
+ * count of the sweeps that did work, [0] for the sweeps that serve L and L~,
+ * [1] for the L-only sweeps. */
+int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double *dts, int32_t n_steps,
+                                void *workspace, size_t workspace_bytes, void *stream, double *sweep_ms,
+                                int32_t *launches);
+
 /* ------------------------------------------------------------------ 3-D WENO5
  * The 3-D characteristic WENO5 step of code/snapshot/warp_weno5_3d_rk3:
  * state (nz+6, ny+6, nx+6, 5) FP64 conserved (rho, rho u, rho v, rho w, E),

@@ -22,7 +22,7 @@ EXPORTS = (
     "wsr_euler2d_loop_steps_timed", "wsr_euler2d_counters", "wsr_adv1d_rollout",
     "wsr_adv1d_train_forward", "wsr_adv1d_train_adjoint",
     "wsr_euler7_workspace_bytes", "wsr_euler7_rhs", "wsr_euler7_step", "wsr_euler7_loop_init",
-    "wsr_euler7_loop_steps",
+    "wsr_euler7_loop_steps", "wsr_euler7_loop_steps_timed",
     "wsr_euler3_workspace_bytes", "wsr_euler3_step", "wsr_euler3_max_speed", "wsr_euler3_loop_init",
     "wsr_euler3_loop_steps", "wsr_euler3_loop_steps_timed",
 )
@@ -134,6 +134,8 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler7_loop_init.argtypes = [C7, P, I32, P, SZ, P]
         L.wsr_euler7_loop_steps.restype = ctypes.c_int
         L.wsr_euler7_loop_steps.argtypes = [C7, P, P, I32, P, SZ, P]
+        L.wsr_euler7_loop_steps_timed.restype = ctypes.c_int
+        L.wsr_euler7_loop_steps_timed.argtypes = [C7, P, P, I32, P, SZ, P, ctypes.POINTER(D), ctypes.POINTER(I32)]
         C3 = ctypes.POINTER(Euler3Config)
         L.wsr_euler3_workspace_bytes.restype = SZ
         L.wsr_euler3_workspace_bytes.argtypes = [C3]

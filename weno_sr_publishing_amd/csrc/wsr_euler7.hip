@@ -527,8 +527,26 @@ static void launch7(const wsr_euler7_config *cfg, const double *bc, double *F, d
 }
 
 // boundary fill of src at time (host value, or ctl->t + frac * dt) and both sweeps
+// optional HIP-event pairs around the sweep launches (kind 0: L and L~ sweep, 1: L only)
+constexpr int TIMER7_MAX_STEPS = 32;
+struct Timer7 {
+    hipEvent_t ev[2 * 8 * TIMER7_MAX_STEPS];
+    int kind[8 * TIMER7_MAX_STEPS];
+    int used = 0, cap = 0;
+    void begin(hipStream_t st) {
+        if (used < cap) (void)hipEventRecord(ev[2 * used], st);
+    }
+    void end(int k, hipStream_t st) {
+        if (used < cap) {
+            (void)hipEventRecord(ev[2 * used + 1], st);
+            kind[used] = k;
+            used += 1;
+        }
+    }
+};
+
 static void enqueue_rhs7(const wsr_euler7_config *cfg, const Workspace7 &w, const double *src, bool both,
-                         double dt, double time, const Ctl *ctl, double frac, hipStream_t st) {
+                         double dt, double time, const Ctl *ctl, double frac, hipStream_t st, Timer7 *tm = nullptr) {
     Grid g = grid7(cfg);
     const int np = g.nxt * g.nyt;
     hipLaunchKernelGGL(k7_boundary, dim3((np + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, src, w.bc, g,
@@ -537,26 +555,30 @@ static void enqueue_rhs7(const wsr_euler7_config *cfg, const Workspace7 &w, cons
     A.g = g; A.dt_host = dt; A.ctl = ctl; A.eno = cfg->eno_cutoff;
     A.z_fallback = (cfg->flags & WSR_FLAG_Z_FALLBACK) ? 1 : 0;
     A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
+    if (tm) tm->begin(st);
     launch7<1>(cfg, w.bc, w.Fx, both ? w.Fxt : nullptr, A, w.mlp, st);
+    if (tm) tm->end(both ? 0 : 1, st);
+    if (tm) tm->begin(st);
     launch7<2>(cfg, w.bc, w.Fy, both ? w.Fyt : nullptr, A, w.mlp, st);
+    if (tm) tm->end(both ? 0 : 1, st);
 }
 
 // one Shu RK4 step of u (host dt/time, or the device-resident ones when ctl != nullptr)
 static void enqueue_step7(const wsr_euler7_config *cfg, const Workspace7 &w, double *u, double dt, double time,
-                          Ctl *ctl, double *dts, hipStream_t st) {
+                          Ctl *ctl, double *dts, hipStream_t st, Timer7 *tm = nullptr) {
     Grid g = grid7(cfg);
     const int nc = g.nx * g.ny;
     dim3 grid((nc + BLOCK - 1) / BLOCK);
     StageArrays S;
     S.u = u; S.u1 = w.u1; S.u2 = w.u2; S.u3 = w.u3; S.rt0 = w.rt0; S.acc = w.acc;
     S.Fx = w.Fx; S.Fy = w.Fy; S.Fxt = w.Fxt; S.Fyt = w.Fyt; S.dts = dts;
-    enqueue_rhs7(cfg, w, u, true, dt, time, ctl, 0.0, st);
+    enqueue_rhs7(cfg, w, u, true, dt, time, ctl, 0.0, st, tm);
     hipLaunchKernelGGL(k7_stage<0>, grid, dim3(BLOCK), 0, st, S, g, dt, ctl, 0);
-    enqueue_rhs7(cfg, w, w.u1, true, dt, time + 0.5 * dt, ctl, 0.5, st);
+    enqueue_rhs7(cfg, w, w.u1, true, dt, time + 0.5 * dt, ctl, 0.5, st, tm);
     hipLaunchKernelGGL(k7_stage<1>, grid, dim3(BLOCK), 0, st, S, g, dt, ctl, 0);
-    enqueue_rhs7(cfg, w, w.u2, false, dt, time + 0.5 * dt, ctl, 0.5, st);
+    enqueue_rhs7(cfg, w, w.u2, false, dt, time + 0.5 * dt, ctl, 0.5, st, tm);
     hipLaunchKernelGGL(k7_stage<2>, grid, dim3(BLOCK), 0, st, S, g, dt, ctl, 0);
-    enqueue_rhs7(cfg, w, w.u3, false, dt, time + dt, ctl, 1.0, st);
+    enqueue_rhs7(cfg, w, w.u3, false, dt, time + dt, ctl, 1.0, st, tm);
     hipLaunchKernelGGL(k7_stage<3>, grid, dim3(BLOCK), 0, st, S, g, dt, ctl, ctl ? 1 : 0);
 }
 
@@ -637,6 +659,42 @@ int wsr_euler7_loop_steps(const wsr_euler7_config *cfg, double *u, double *dts, 
         enqueue_step7(cfg, w, u, 0.0, 0.0, w.ctl, dts, st);
     }
     return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double *dts, int32_t n_steps,
+                                void *workspace, size_t workspace_bytes, void *stream, double *sweep_ms,
+                                int32_t *launches) {
+    int rc = check7(cfg);
+    if (rc) return rc;
+    if (!u || !dts || !workspace || n_steps < 0 || n_steps > TIMER7_MAX_STEPS || !sweep_ms || !launches)
+        return WSR_E_ARG;
+    if (workspace_bytes < ws7_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
+    Workspace7 w;
+    ws7_layout(cfg->nx, cfg->ny, &w, workspace);
+    hipStream_t st = (hipStream_t)stream;
+    Timer7 *tm = new Timer7();
+    tm->cap = 8 * n_steps;
+    for (int i = 0; i < 2 * tm->cap; ++i)
+        if (hipEventCreate(&tm->ev[i]) != hipSuccess) { delete tm; return WSR_E_HIP; }
+    if (n_steps > 0) enqueue_params7(cfg, w, st);
+    for (int n = 0; n < n_steps; ++n) {
+        hipLaunchKernelGGL(k7_dt, dim3(1), dim3(1), 0, st, w.ctl, cfg->loop_mode);
+        enqueue_step7(cfg, w, u, 0.0, 0.0, w.ctl, dts, st, tm);
+    }
+    rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) rc = WSR_E_HIP;
+    for (int k = 0; k < 2; ++k) { sweep_ms[k] = 0.0; launches[k] = 0; }
+    for (int i = 0; i < tm->used; ++i) {
+        float ms = 0.0f;
+        // launches after the loop stopped return at once; count only the ones that did work
+        if (hipEventElapsedTime(&ms, tm->ev[2 * i], tm->ev[2 * i + 1]) == hipSuccess && ms > 0.01f) {
+            sweep_ms[tm->kind[i]] += ms;
+            launches[tm->kind[i]] += 1;
+        }
+    }
+    for (int i = 0; i < 2 * tm->cap; ++i) (void)hipEventDestroy(tm->ev[i]);
+    delete tm;
+    return rc;
 }
 
 }  // extern "C"

@@ -157,6 +157,17 @@ class Euler7Solver:
                                              self.workspace.data_ptr(), self.ws_bytes, self._stream())
         _lib.check(rc, "wsr_euler7_loop_steps")
 
+    def loop_steps_timed(self, u: torch.Tensor, n: int) -> tuple[list[float], list[int]]:
+        """n (<= 32) steps with HIP events around every sweep launch: ([ms] x 2,
+        [launches] x 2) for the L-and-L~ sweeps and the L-only sweeps that did work."""
+        ms = (ctypes.c_double * 2)()
+        cnt = (ctypes.c_int32 * 2)()
+        rc = self._lib.wsr_euler7_loop_steps_timed(ctypes.byref(self.cfg), u.data_ptr(), self._dts.data_ptr(),
+                                                   int(n), self.workspace.data_ptr(), self.ws_bytes, self._stream(),
+                                                   ms, cnt)
+        _lib.check(rc, "wsr_euler7_loop_steps_timed")
+        return [float(v) for v in ms], [int(v) for v in cnt]
+
     def loop_status(self) -> tuple[int, float, float, bool]:
         steps, t, dt, done = ctypes.c_int32(0), ctypes.c_double(0.0), ctypes.c_double(0.0), ctypes.c_int32(0)
         rc = self._lib.wsr_euler2d_loop_status(self.workspace.data_ptr(), ctypes.byref(steps), ctypes.byref(t),
