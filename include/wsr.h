@@ -70,7 +70,7 @@ typedef struct wsr_euler2d_config {
     double cfl, t_end;              /* used by the device-resident time loop only */
     /* host pointer to the 189 closure parameters w1,b1,w2,b2,w3,b3,w4,b4
      * (row-major, no singleton axis): double for SR64, float for SR32/SRBF16;
-     * NULL for JS and Z.  Copied into the kernel arguments at every launch. */
+     * NULL for JS and Z.  Copied into the workspace by each stage / loop call. */
     const void *mlp;
 } wsr_euler2d_config;
 

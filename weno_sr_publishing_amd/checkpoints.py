@@ -85,7 +85,8 @@ def load_mlp_params(model_path, device: str = "cuda", precision: str | None = No
     ``precision`` defaults to the checkpoint's own dtype (FP64 checkpoints give
     the FP64 closure, FP32 checkpoints the FP32 closure); pass "bf16" for the
     BF16 closure.  ``device`` is accepted for signature compatibility: the
-    kernels receive the 189 parameters by value, so nothing is copied to the GPU.
+    parameters stay on the host until a solver call stages them into its
+    device workspace.
     """
     del device
     arrays, meta = _read(model_path)

@@ -15,9 +15,10 @@
 //   k_update   : u_out = SSPRK3 combination of u0 and u - (dFx + dFy)/2; in the
 //                device time loop it also reduces the next step's max speed.
 // Each face is owned by eight lanes (side s = 0/1 x component k = 0..3); the
-// lanes of a face exchange values with cross-lane shuffles.  The MLP weights
-// travel in the kernel arguments (189 FP32/FP64 values), so they are read
-// with scalar loads and shared by the whole wavefront.
+// lanes of a face exchange values with cross-lane shuffles.  The 189 MLP
+// weights are staged once per call into the workspace (k_put_params) and read
+// through a constant-address-space pointer: scalar loads shared by the whole
+// wavefront, fenced per neuron so that they are not hoisted into SGPR spills.
 //
 // Reference (paths relative to the reference checkout):
 //   launch order / stage: for_paper_results/solvers/weno5_hllc_mixed.py:121-168

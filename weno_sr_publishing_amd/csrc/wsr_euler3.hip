@@ -388,7 +388,7 @@ __global__ __launch_bounds__(BLOCK) void k3_normal(const double *__restrict__ q,
 #pragma unroll
         for (int c = 0; c < 5; ++c) dst[c * g.np + base] = out.v[c];
     }
-    if (A.counters) cnt.flush(A.counters);
+    if (A.counters) cnt.flush_block(A.counters);
 }
 
 // first transverse Gauss pass (kernels.py:941-1014): lane -> (side, comp, position)
@@ -430,7 +430,7 @@ __global__ __launch_bounds__(BLOCK) void k3_trans1(const double *__restrict__ NL
     if (A.counters) {
         LaneCounts cnt;
         cnt.add(2, 5, active, ev);
-        cnt.flush(A.counters);
+        cnt.flush_block(A.counters);
     }
 }
 
@@ -501,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k3_flux(const double *__restrict__ S, d
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[c * g.np + pos] = tot[c];
     }
-    if (A.counters) cnt.flush(A.counters);
+    if (A.counters) cnt.flush_block(A.counters);
 }
 
 // kernels.py:1092-1163 rk_stage{1,2,3}: q <- {adv, 3/4 q0 + 1/4 adv, 1/3 q0 + 2/3 adv},

@@ -58,6 +58,21 @@ struct LaneCounts {
             if ((threadIdx.x & 63) == 0 && v) atomicAdd(dst + i, (unsigned long long)v);
         }
     }
+    // the same sums gathered per block in LDS first: one global atomic per block
+    // and counter (every thread of the block must call it)
+    __device__ __forceinline__ void flush_block(unsigned long long *dst) {
+        __shared__ unsigned blk[NCOUNT];
+        if (threadIdx.x < NCOUNT) blk[threadIdx.x] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NCOUNT; ++i) {
+            unsigned v = c[i];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+            if ((threadIdx.x & 63) == 0 && v) atomicAdd(blk + i, v);
+        }
+        __syncthreads();
+        if (threadIdx.x < NCOUNT && blk[threadIdx.x]) atomicAdd(dst + threadIdx.x, (unsigned long long)blk[threadIdx.x]);
+    }
 };
 
 __device__ __forceinline__ double cell_speed(const double q[4], double gamma) {
