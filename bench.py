@@ -454,16 +454,250 @@ def bench_doublemach7(args) -> int:
     return 0
 
 
+# ------------------------------------------------------------- 1-D workloads
+# Operation counts per cell and SSPRK3 stage of the 1-D signed advection step: one
+# face reconstruction (NORMAL above) + flux a*face, difference, stage combination.
+ADV_STAGE_FIXED = 8
+# Reverse pass of one SR reconstruction in the adjoint (recomputed forward MLP pair
+# 2 x 415, backward through the pair 2 x 810: both products of every dense layer plus
+# the swish / tanh / softmax derivatives, feature and weight derivatives 80).
+SR_RECON_BWD = 2 * MLP_RAW + 2 * 810 + 80
+PLATEAU_RECON_BWD = 60
+
+
+def plateau_fraction_1d(u: np.ndarray) -> float:
+    """Share of periodic 5-cell stencils that are plateaus (delta_max <= 1e-13 q_scale)."""
+    q = [np.roll(u, 2 - m, axis=-1) for m in range(5)]
+    d0 = 13.0 / 12.0 * np.abs(q[0] - 2 * q[1] + q[2]) + 0.25 * np.abs(q[0] - 4 * q[1] + 3 * q[2])
+    d1 = 13.0 / 12.0 * np.abs(q[1] - 2 * q[2] + q[3]) + 0.25 * np.abs(q[1] - q[3])
+    d2 = 13.0 / 12.0 * np.abs(q[2] - 2 * q[3] + q[4]) + 0.25 * np.abs(3 * q[2] - 4 * q[3] + q[4])
+    scale = np.maximum(np.max(np.abs(np.stack(q)), axis=0), 1.0)
+    return float(np.mean(np.maximum(np.maximum(d0, d1), d2) <= 1e-13 * scale))
+
+
+def bench_train1d(args) -> int:
+    import torch
+    from weno_sr_publishing_amd.profiles import make_batch, shifted_target
+    from weno_sr_publishing_amd.training import ClosureTrainer, distance_schedule
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}), flush=True)
+        return 1
+    dev = torch.device("cuda:0")
+    B, N, DIST, CFL = 512, 256, 64, 0.6
+    u0, vel = make_batch(B, N, seed=83)
+    target = shifted_target(u0, vel, DIST)
+    dts = distance_schedule(N, DIST, CFL)
+    nsteps = len(dts)
+    u0_t = torch.tensor(u0, device=dev)
+    vel_t = torch.tensor(vel, device=dev)
+    tgt_t = torch.tensor(target, device=dev)
+    trainer = ClosureTrainer(precision="fp32", lr=1e-3, device=dev)
+    iters = args.steps if args.steps > 0 else 50
+    for _ in range(args.warmup):
+        trainer.step(u0_t, vel_t, tgt_t, dts)
+    torch.cuda.synchronize()
+    losses = []
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        losses.append(trainer.step(u0_t, vel_t, tgt_t, dts))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the same iterations with CUDA events around the forward and adjoint launches
+    events: dict = {}
+    for _ in range(iters):
+        loss, grad, _ = trainer.loss_and_grad(u0_t, vel_t, tgt_t, dts, events=events)
+        trainer.param.grad.copy_(grad.to(trainer.dtype))
+        trainer.opt.step()
+    torch.cuda.synchronize()
+
+    def avg_ms(name):
+        ev = events[name]
+        return sum(ev[i].elapsed_time(ev[i + 1]) for i in range(0, len(ev), 2)) / (len(ev) // 2)
+
+    fwd_ms, adj_ms = avg_ms("forward"), avg_ms("adjoint")
+    frac = 1.0 - plateau_fraction_1d(np.concatenate([u0, target]))
+    recon = NORMAL["sr"][0] + frac * (NORMAL["sr"][1] - NORMAL["sr"][0])
+    fwd_flops = B * N * nsteps * 3 * (recon + ADV_STAGE_FIXED)
+    bwd_recon = PLATEAU_RECON_BWD + frac * (SR_RECON_BWD - PLATEAU_RECON_BWD)
+    adj_flops = B * N * nsteps * (2 * (recon + ADV_STAGE_FIXED) + 3 * (bwd_recon + 2 * ADV_STAGE_FIXED))
+    achieved = adj_flops / (adj_ms * 1e-3) / 1e12
+    cell_steps = B * N * nsteps
+    value = cell_steps * iters / wall
+    out = {
+        "metric": "closure training throughput: trajectory cell-updates/s (forward rollout + hand-written adjoint "
+                  "+ Adam)",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": iters,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / iters,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: 512 random periodic profiles (Fourier, Gaussian, square, triangle, semi-ellipse, "
+                "composites; exact cell averages, seed 83), a = +1 / -1 balanced, targets = exact shifted averages",
+        "config": {"workload": f"1D closure training, batch {B}, N={N}, 64-cell rollout at CFL 0.6 ({nsteps} "
+                               "SSPRK3 steps), MSE loss, FP32 closure gradients", "batch": B, "n": N,
+                   "rollout_steps": nsteps, "closure": "sr32", "state": "fp64"},
+        "loss_first_last": [losses[0], losses[-1]],
+        "roofline": {
+            "bound": "mfma",
+            "compute": "VALU (no MFMA): FP32 closure MLP forward/backward + FP64 state; peak is the MI355X "
+                       "FP64-vector / non-packed FP32-vector rate",
+            "kernel": "k_train_adjoint<sr32>",
+            "achieved": achieved,
+            "peak": PEAK_VALU_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_VALU_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": adj_ms,
+            "launches": iters,
+            "algorithmic_flops_per_launch": adj_flops,
+            "closure_fraction_estimate": frac,
+            "forward_kernel": {"avg_launch_ms": fwd_ms, "algorithmic_flops_per_launch": fwd_flops,
+                               "achieved": fwd_flops / (fwd_ms * 1e-3) / 1e12},
+            "hbm_algorithmic_GBs": (B * (nsteps + 1) * N * 8 * 2) / (adj_ms * 1e-3) / 1e9,
+        },
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        done, t0 = 0, time.perf_counter()
+        while True:
+            oracle.adv1d_run(u0[:32], vel[:32], dts, float(N), scheme="sr32",
+                             mlp=trainer.mlp())
+            done += 32
+            el = time.perf_counter() - t0
+            if el >= args.cpu_seconds or done >= B:
+                break
+        cpu_rate = done * N * nsteps / el
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s (forward rollout only)", "cores": threads,
+                               "kind": "port",
+                               "sample": f"{done} of the 512 trajectories rolled forward 64 cells ({nsteps} steps, "
+                                         f"sr32) by the OpenMP C oracle in {el:.1f} s; the oracle has no adjoint: "
+                                         "a gradient by central differences costs 2 x 189 + 1 such rollouts",
+                               "cpu_model": cpu_model_name(), "host_cpus": os.cpu_count()}
+        out["gpu_vs_cpu"] = value / cpu_rate
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_gste1d(args) -> int:
+    import torch
+    from weno_sr_publishing_amd.advection1d import Adv1DSolver
+    from weno_sr_publishing_amd.checkpoints import load_f32, load_f64
+    from weno_sr_publishing_amd.initial import GSTE_LENGTH, gste_cell_averages
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}), flush=True)
+        return 1
+    dev = torch.device("cuda:0")
+    n, t_end, cfl = 200, 10.0, 0.6
+    dx = GSTE_LENGTH / n
+    steps = int(np.ceil(t_end / (cfl * dx)))
+    dts = [t_end / steps] * steps
+    _, u0 = gste_cell_averages(n, 0.0, 15)
+    _, exact = gste_cell_averages(n, t_end, 15)
+    closure = {"sr32": "sr32", "sr64": "sr64", "srbf16": "sr32"}[args.closure]
+    models = {"sr32": load_f32(), "sr64": load_f64(), "js": None}
+    reps = args.steps if args.steps > 0 else 20
+
+    def run(scheme, batch):
+        solver = Adv1DSolver(n, 1.0 / dx, models[scheme], dev)
+        u_init = torch.tensor(np.tile(u0, (batch, 1)), device=dev)
+        u = u_init.clone()
+        for _ in range(max(1, args.warmup // 10)):
+            u.copy_(u_init)
+            solver.rollout(u, [1.0], dts)
+        torch.cuda.synchronize()
+        ms = []
+        for _ in range(reps):
+            u.copy_(u_init)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            solver.rollout(u, [1.0], dts)
+            b.record()
+            b.synchronize()
+            ms.append(a.elapsed_time(b))
+        return u, float(np.mean(ms))
+
+    u_sr, ms_sr = run(closure, 1)
+    _, ms_js = run("js", 1)
+    ub, ms_b = run(closure, 4096)
+    l1 = float(np.mean(np.abs(u_sr[0].cpu().numpy() - exact)))
+    assert np.allclose(ub[0].cpu().numpy(), u_sr[0].cpu().numpy(), rtol=0, atol=1e-13)
+    frac = 1.0 - plateau_fraction_1d(u0)
+    recon = NORMAL["sr"][0] + frac * (NORMAL["sr"][1] - NORMAL["sr"][0])
+    flops_b = 4096 * n * steps * 3 * (recon + ADV_STAGE_FIXED)
+    value = n * steps / (ms_sr * 1e-3)
+    out = {
+        "metric": "1D advection cell-updates/s, GSTE (N=200, t=10, CFL 0.6), WENO5-SR",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_sr / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: GSTE profile (Gaussian, square, triangle, semi-ellipse), 15-point Gauss-Legendre cell "
+                "averages",
+        "config": {"workload": f"1D periodic advection GSTE N={n}, t={t_end}, CFL {cfl} ({steps} SSPRK3 steps), "
+                               f"WENO5-SR ({closure} closure); one trajectory per launch", "n": n,
+                   "closure": closure},
+        "rollout_ms": ms_sr,
+        "l1_vs_exact": l1,
+        "js": {"rollout_ms": ms_js, "value": n * steps / (ms_js * 1e-3)},
+        "batched_4096": {"rollout_ms": ms_b, "value": 4096 * n * steps / (ms_b * 1e-3)},
+        "roofline": {
+            "bound": "mfma",
+            "compute": "VALU; the single trajectory occupies one workgroup on one CU (latency-bound): the roofline "
+                       "figure is that of 4096 copies of the run in one launch",
+            "kernel": f"k_adv1d_rollout<{closure}> (batch 4096)",
+            "achieved": flops_b / (ms_b * 1e-3) / 1e12,
+            "peak": PEAK_VALU_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": flops_b / (ms_b * 1e-3) / 1e12 / PEAK_VALU_TFLOPS,
+            "traffic": None,
+            "avg_launch_ms": ms_b,
+            "algorithmic_flops_per_launch": flops_b,
+            "closure_fraction_estimate": frac,
+        },
+    }
+    if not args.no_cpu:
+        from oracle import oracle
+        oracle.set_threads(1)
+        t0 = time.perf_counter()
+        oracle.adv1d_run(u0[None, :], [1.0], dts, 1.0 / dx, scheme=closure, mlp=models[closure])
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole GSTE run ({steps} steps, {closure}) by the C oracle on one "
+                                         f"thread, {el:.2f} s", "cpu_model": cpu_model_name(),
+                               "host_cpus": os.cpu_count()}
+        out["gpu_vs_cpu"] = value / (n * steps / el)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed SSPRK3 steps (0 = full trajectory to t=0.25)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--closure", default="sr32", choices=["sr32", "sr64", "srbf16"])
-    ap.add_argument("--workload", default="c4-400", choices=["c4-400", "shockbubble3d", "doublemach7"],
+    ap.add_argument("--workload", default="c4-400",
+                    choices=["c4-400", "shockbubble3d", "doublemach7", "train1d", "gste1d"],
                     help="c4-400: the headline 2-D line; shockbubble3d: the 3-D 224x88x88 Mach-3 "
                          "shock-bubble leg (200 fixed steps); doublemach7: WENO7 + Shu RK4 double Mach "
-                         "reflection at 1200x300 to t=0.2; each prints its own JSON line")
+                         "reflection at 1200x300 to t=0.2; train1d: closure training iterations (512 "
+                         "trajectories x 256 cells, 64-cell rollout, hand-written adjoint); gste1d: the 1-D "
+                         "GSTE advection run (N=200, t=10); each prints its own JSON line")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
@@ -486,6 +720,10 @@ def main() -> int:
         return bench_shockbubble3d(args)
     if args.workload == "doublemach7":
         return bench_doublemach7(args)
+    if args.workload == "train1d":
+        return bench_train1d(args)
+    if args.workload == "gste1d":
+        return bench_gste1d(args)
 
     import torch
     from weno_sr_publishing_amd.checkpoints import load_f32, load_f64

@@ -80,8 +80,10 @@ class ClosureTrainer:
         return _lib.Adv1DConfig(batch=batch, n=n, scheme=self.scheme, reserved=0, dxinv=dxinv, mlp=None)
 
     def loss_and_grad(self, u0: torch.Tensor, velocities: torch.Tensor, target: torch.Tensor, dts,
-                      return_state_grad: bool = False):
-        """Forward rollout, loss and parameter gradient (summed over the batch)."""
+                      return_state_grad: bool = False, events: dict | None = None):
+        """Forward rollout, loss and parameter gradient (summed over the batch).
+        ``events``, if given, receives CUDA event pairs recorded on the launch
+        stream around the forward and adjoint kernels ("forward", "adjoint")."""
         batch, n = u0.shape
         dxinv = float(n)
         u = u0.detach().to(self.device, torch.float64).contiguous().clone()
@@ -92,8 +94,17 @@ class ClosureTrainer:
         stream = torch.cuda.current_stream(self.device).cuda_stream
         cfg = self._cfg(batch, n, dxinv)
         p = self.param.detach().contiguous()
+
+        def mark(name):
+            if events is not None:
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record(torch.cuda.current_stream(self.device))
+                events.setdefault(name, []).append(ev)
+
+        mark("forward")
         rc = self._lib.wsr_adv1d_train_forward(ctypes.byref(cfg), u.data_ptr(), vel.data_ptr(), dts_t.data_ptr(),
                                                steps, ck.data_ptr(), p.data_ptr(), stream)
+        mark("forward")
         _lib.check(rc, "wsr_adv1d_train_forward")
         tgt = target.to(self.device, torch.float64)
         diff = u - tgt
@@ -101,10 +112,13 @@ class ClosureTrainer:
         lam = (2.0 / diff.numel()) * diff
         grad_b = torch.empty((batch, 189), dtype=torch.float64, device=self.device)
         lam0 = torch.empty((batch, n), dtype=torch.float64, device=self.device) if return_state_grad else None
-        rc = self._lib.wsr_adv1d_train_adjoint(ctypes.byref(cfg), ck.data_ptr(), lam.contiguous().data_ptr(),
+        lam = lam.contiguous()
+        mark("adjoint")
+        rc = self._lib.wsr_adv1d_train_adjoint(ctypes.byref(cfg), ck.data_ptr(), lam.data_ptr(),
                                                vel.data_ptr(), dts_t.data_ptr(), steps, p.data_ptr(),
                                                grad_b.data_ptr(), lam0.data_ptr() if lam0 is not None else None,
                                                stream)
+        mark("adjoint")
         _lib.check(rc, "wsr_adv1d_train_adjoint")
         grad = grad_b.sum(dim=0)
         if return_state_grad:
