@@ -46,11 +46,15 @@ constexpr int BLOCK = 256;   // element-wise kernels
 #ifndef WSR_SWEEP_TT
 #define WSR_SWEEP_TT 28
 #endif
+#ifndef WSR_SWEEP_SPLIT
+#define WSR_SWEEP_SPLIT 2
+#endif
 constexpr int TN = WSR_SWEEP_TN;   // faces per tile along the sweep axis
 constexpr int TT = WSR_SWEEP_TT;   // transverse lines per tile (phase 1 covers TT + 4)
 constexpr int NT1 = TT + 4;
 constexpr int NN = TN + 5;         // stencil cells per transverse line in a tile
-constexpr int SBLOCK = NT1 * TN;   // sweep threads: one face-line of phase 1 each
+constexpr int SPLIT = WSR_SWEEP_SPLIT;     // threads per phase-1 face-line (2: twice the waves, half the passes)
+constexpr int SBLOCK = NT1 * TN * SPLIT;   // sweep threads
 static_assert(SBLOCK % 64 == 0 && SBLOCK <= 1024, "sweep block must be whole waves");
 // ---------------------------------------------------------------- boundary
 __global__ __launch_bounds__(BLOCK) void k_boundary(const double *__restrict__ src, double *__restrict__ dst,
@@ -131,10 +135,16 @@ constexpr int FV_SIZE = NT1 * TN * 8;
 constexpr int P1_SIZE = NN * NT1 * 4 + N1 * 4;
 constexpr int P2_SIZE = CH * 16 + CH * 8;
 constexpr int REG_SIZE = P1_SIZE > P2_SIZE ? P1_SIZE : P2_SIZE;
-static_assert(N1 == SBLOCK, "phase 1a assigns one face-line per thread");
+static_assert(N1 * SPLIT == SBLOCK, "phase 1a assigns one face-line per thread");
+static_assert(N1 % (SBLOCK / 8) == 0, "phase 1b passes cover the face-lines exactly");
+
+// Waves per SIMD the register allocation aims for: the tiles of a 400^2 sweep fit
+// on the chip in about one round, so latency hiding comes from resident waves;
+// the FP64 closure keeps the default (its MLP would spill below ~160 VGPRs).
+template <int SCHEME> struct SweepWaves { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : 5; };
 
 template <int DIR, int SCHEME, int RIEMANN>
-__global__ __launch_bounds__(SBLOCK) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
+__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWaves<SCHEME>::value))) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
                                                  SweepArgs A, typename MlpSweep<SCHEME>::type mlp) {
     double dt = A.dt_host;
     if (A.ctl) {
@@ -172,7 +182,7 @@ __global__ __launch_bounds__(SBLOCK) void k_sweep(const double *__restrict__ bc,
     __syncthreads();
 
     // ---------------- phase 1a: Roe average and eigenvalues, one face-line per thread
-    {
+    if (threadIdx.x < N1) {
         int fl, tau;
         item_map<DIR>(threadIdx.x, NT1, fl, tau);
         Eigen E = roe_eigen(T + ((fl + 2) * NT1 + tau) * 4, T + ((fl + 3) * NT1 + tau) * 4, g.gamma);
@@ -185,7 +195,7 @@ __global__ __launch_bounds__(SBLOCK) void k_sweep(const double *__restrict__ bc,
     const int gi = lane & 7;
     const int s = gi >> 2;           // 0: state left of the face (right face of cell f-1), 1: right state
     const int k = gi & 3;            // characteristic / conserved component
-    const int slot = lane >> 3;      // 32 face items per pass
+    const int slot = lane >> 3;      // SBLOCK / 8 face items per pass
 
     // ---------------- phase 1b: characteristic projection, WENO face value, back-projection
     LaneCounts cnt;

@@ -48,6 +48,16 @@ namespace e3 {
 
 constexpr int BLOCK = 256;
 constexpr int GC = 3;
+#ifndef WSR3_WAVES_NORMAL
+#define WSR3_WAVES_NORMAL 1
+#endif
+#ifndef WSR3_WAVES_FLUX
+#define WSR3_WAVES_FLUX 1
+#endif
+// waves per SIMD the register allocation of the normal / flux passes aims for
+// (the FP64 closure keeps the default)
+template <int SCHEME> struct WavesNormal { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : WSR3_WAVES_NORMAL; };
+template <int SCHEME> struct WavesFlux { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : WSR3_WAVES_FLUX; };
 
 struct Grid3 {
     int nx, ny, nz, nxt, nyt, nzt;
@@ -315,7 +325,7 @@ __global__ __launch_bounds__(BLOCK) void k3_boundary(double *__restrict__ q, Gri
 // blockIdx.y = 0: LR = 2 (face left of the stencil centre) -> NL,
 // blockIdx.y = 1: LR = 1 on R (L q) (face right of the centre) -> NR
 template <int DIR, int SCHEME>
-__global__ __launch_bounds__(BLOCK) void k3_normal(const double *__restrict__ q, double *__restrict__ NL,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WavesNormal<SCHEME>::value))) void k3_normal(const double *__restrict__ q, double *__restrict__ NL,
                                                    double *__restrict__ NR, PassArgs<SCHEME> A) {
     if (A.ctl && A.ctl->done) return;
     const Grid3 &g = A.g;
@@ -437,7 +447,7 @@ __global__ __launch_bounds__(BLOCK) void k3_trans1(const double *__restrict__ NL
 // second transverse pass + Riemann solves + flag sum (solver.py:176-203,
 // kernels.py:1016-1090); lanes 2f, 2f+1 own face f with first_lr = 1, 2
 template <int DIR, int SCHEME, int RIEMANN>
-__global__ __launch_bounds__(BLOCK) void k3_flux(const double *__restrict__ S, double *__restrict__ F,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WavesFlux<SCHEME>::value))) void k3_flux(const double *__restrict__ S, double *__restrict__ F,
                                                  PassArgs<SCHEME> A, double dt_host) {
     if (A.ctl && A.ctl->done) return;
     const Grid3 &g = A.g;

@@ -34,15 +34,25 @@
 namespace wsr {
 namespace w7 {
 
+#ifndef WSR7_SPLIT
+#define WSR7_SPLIT 2
+#endif
+#ifndef WSR7_WAVES
+#define WSR7_WAVES 3
+#endif
 constexpr int GC = 4;
 constexpr int BLOCK = 256;
 constexpr int TN = 4;              // faces per tile along the sweep axis
 constexpr int TT = 26;             // transverse flux lines per tile (phase 1 covers TT + 6)
 constexpr int NT1 = TT + 6;
 constexpr int NN = TN + 7;         // stencil cells per transverse line
-constexpr int SBLOCK = NT1 * TN;   // 128 threads: one face-line of phase 1 each
+constexpr int SPLIT = WSR7_SPLIT;  // threads per phase-1 face-line (2: twice the waves, half the passes)
+constexpr int SBLOCK = NT1 * TN * SPLIT;
 static_assert(SBLOCK % 64 == 0, "whole waves");
 constexpr int N1 = NT1 * TN;
+static_assert(N1 % (SBLOCK / 8) == 0, "phase 1b passes cover the face-lines exactly");
+// waves per SIMD the register allocation aims for (the FP64 closure keeps the default)
+template <int SCHEME> struct Sweep7Waves { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : WSR7_WAVES; };
 constexpr int N2 = TT * TN;
 constexpr int CH = SBLOCK / 2;
 constexpr int FV_SIZE = NT1 * TN * 8;
@@ -127,7 +137,7 @@ __device__ __forceinline__ void riemann7(const double ul[4], const double ur[4],
 // F[face] = F(g1) + F(g2) with the states (left, right); with BOTH also
 // Ft[face] = F(right, left) at both points (the downwind operator).
 template <int DIR, int SCHEME, int RIEMANN, bool BOTH>
-__global__ __launch_bounds__(SBLOCK) void k7_sweep(const double *__restrict__ bc, double *__restrict__ F,
+__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Waves<SCHEME>::value))) void k7_sweep(const double *__restrict__ bc, double *__restrict__ F,
                                                    double *__restrict__ Ft, SweepArgs7 A,
                                                    typename Mlp7Sweep<SCHEME>::type mlp) {
     double dt = A.dt_host;
@@ -167,7 +177,7 @@ __global__ __launch_bounds__(SBLOCK) void k7_sweep(const double *__restrict__ bc
     __syncthreads();
 
     // phase 1a: Roe average of the two cells next to each face (cells n = fl+3, fl+4)
-    {
+    if (threadIdx.x < N1) {
         int fl, tau;
         item_map<DIR>(threadIdx.x, NT1, fl, tau);
         Eigen E = roe_eigen(T + ((fl + 3) * NT1 + tau) * 4, T + ((fl + 4) * NT1 + tau) * 4, g.gamma);
