@@ -12,7 +12,9 @@
 //                     stages: per face the VJP of the reconstruction (wsr_adjoint.hpp)
 //                     gives the stencil adjoints (gathered per cell, no atomics)
 //                     and two parameter-gradient rows per MLP evaluation; the rows
-//                     of a pass are reduced in LDS by one thread per parameter.
+//                     of a pass are reduced on the matrix cores (16x16x4 MFMA outer
+//                     products, FP32 rows for the FP32 closure, FP64 for FP64) and
+//                     the per-pass tiles accumulated in FP64.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -76,15 +78,76 @@ __device__ __forceinline__ void faces_into(const double *src, double *dst, int n
     for (int i = threadIdx.x; i < n; i += nthreads) dst[i] = a * face_signed<SCHEME>(src, n, i, a, P);
 }
 
-// VJP of R(X)_j = -(F_j - F_{j-1}) dxinv with cotangent g: target += R_X^T g, acc += R_theta^T g.
-// parameters owned by one thread of an NT-thread workgroup: p = tid, tid + NT, ...
-constexpr int OWN = 2;
+// Parameter gradients as matrix-core outer products.  d(loss)/d(param) is a sum
+// over gradient rows (one per MLP evaluation) of row[in] * row[out]; grouped by
+// layer these are the products  IN^T OUT  of the rows' input activations and
+// output adjoints, accumulated by 16 x 16 x 4 MFMA tiles over 4 rows at a time:
+//   tile 0: i 0..5  = (x, 1) x j 0..9   = z1bar   (layer 1 weights, bias)
+//           i 6..12 = (h2, 1) x j 10..15 = z3bar  (layer 3)
+//   tile 1: i 0..10 = (h1, 1) x j 0..5  = z2bar   (layer 2)
+//   tile 2: i 0..6  = (h3, 1) x j 0..2  = z4bar   (layer 4)
+// (the other entries of tile 0 are cross terms and are never read).
+constexpr int NTILE = 3;
 
+__device__ __forceinline__ int tile_in_slot(int t, int i) {
+    if (t == 0) {
+        if (i < 5) return R_X + i;
+        if (i == 5) return R_ONE;
+        if (i < 12) return R_H2 + (i - 6);
+        return i == 12 ? R_ONE : -1;
+    }
+    if (t == 1) return i < 10 ? R_H1 + i : (i == 10 ? R_ONE : -1);
+    return i < 6 ? R_H3 + i : (i == 6 ? R_ONE : -1);
+}
+__device__ __forceinline__ int tile_out_slot(int t, int j) {
+    if (t == 0) return j < 10 ? R_Z1 + j : R_Z3 + (j - 10);
+    if (t == 1) return j < 6 ? R_Z2 + j : -1;
+    return j < 3 ? R_Z4 + j : -1;
+}
+// (tile, i, j) of parameter p (inverse of param_row_slots)
+__device__ __forceinline__ void param_tile(int p, int &t, int &i, int &j) {
+    if (p < 50) { t = 0; i = p / 10; j = p % 10; }
+    else if (p < 60) { t = 0; i = 5; j = p - 50; }
+    else if (p < 120) { t = 1; i = (p - 60) / 6; j = (p - 60) % 6; }
+    else if (p < 126) { t = 1; i = 10; j = p - 120; }
+    else if (p < 162) { t = 0; i = 6 + (p - 126) / 6; j = 10 + (p - 126) % 6; }
+    else if (p < 168) { t = 0; i = 12; j = 10 + (p - 162); }
+    else if (p < 186) { t = 2; i = (p - 168) / 3; j = (p - 168) % 3; }
+    else { t = 2; i = 6; j = p - 186; }
+}
+
+typedef float mf4 __attribute__((ext_vector_type(4)));
+typedef double md4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Tile;
+template <> struct Tile<float> {
+    using V = mf4;
+    static __device__ __forceinline__ V mfma(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // accumulator register r of lane l holds D[4 (l / 16) + r][l % 16]
+    static __device__ __forceinline__ int row_of(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+template <> struct Tile<double> {
+    using V = md4;
+    static __device__ __forceinline__ V mfma(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // accumulator register r of lane l holds D[(l / 16) + 4 r][l % 16]
+    static __device__ __forceinline__ int row_of(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+
+// VJP of R(X)_j = -(F_j - F_{j-1}) dxinv with cotangent g: target += R_X^T g and the
+// parameter-gradient rows of the pass folded into acc (FP64, lane layout of Tile<T>).
 template <int SCHEME, int NT, typename T, class MP>
 __device__ void vjp_rhs(const double *X, const double *g, double *target, double *adjq, T *rows, int *used, int n,
-                        double a, double dxinv, const MP &P, const int (&my_in)[OWN], const int (&my_out)[OWN],
-                        double (&acc)[OWN]) {
+                        double a, double dxinv, const MP &P, double (&acc)[NTILE][4]) {
     const int off = a > 0.0 ? 2 : 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // operand slots of this lane: A[i][k] = row_k[in(i)], B[k][j] = row_k[out(j)], i = j = lane % 16
+    int sin[NTILE], sout[NTILE];
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) { sin[t] = tile_in_slot(t, lane & 15); sout[t] = tile_out_slot(t, lane & 15); }
     for (int base = 0; base < n; base += NT) {
         const int i = base + threadIdx.x;
         if (i < n) {
@@ -105,18 +168,25 @@ __device__ void vjp_rhs(const double *X, const double *g, double *target, double
             used[threadIdx.x] = 0;
         }
         __syncthreads();
-        // each thread sums the rows of this pass for the parameters it owns
+        // this wave's 64 faces = 128 rows, 4 rows per MFMA step (row k of a step: lane / 16)
+        typename Tile<T>::V d[NTILE];
 #pragma unroll
-        for (int o = 0; o < OWN; ++o) {
-            if (my_in[o] < 0) continue;
-            double s = 0.0;
-            for (int r = 0; r < NT; ++r) {
-                if (!used[r]) continue;
-                const T *rd = rows + (size_t)(2 * r) * ROW;
-                s += (double)rd[my_in[o]] * (double)rd[my_out[o]] + (double)rd[ROW + my_in[o]] * (double)rd[ROW + my_out[o]];
+        for (int t = 0; t < NTILE; ++t) d[t] = typename Tile<T>::V{T(0), T(0), T(0), T(0)};
+        for (int kc = 0; kc < 32; ++kc) {
+            const int r = wave * 128 + kc * 4 + (lane >> 4);
+            const T *row = rows + (size_t)r * ROW;
+            const bool u = used[r >> 1] != 0;
+#pragma unroll
+            for (int t = 0; t < NTILE; ++t) {
+                T av = (u && sin[t] >= 0) ? row[sin[t]] : T(0);
+                T bv = (u && sout[t] >= 0) ? row[sout[t]] : T(0);
+                d[t] = Tile<T>::mfma(av, bv, d[t]);
             }
-            acc[o] += s;
         }
+#pragma unroll
+        for (int t = 0; t < NTILE; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[t][q] += (double)d[t][q];
         __syncthreads();
     }
     // gather the stencil adjoints per cell: cell j is entry m of face j + off - m
@@ -149,15 +219,10 @@ __global__ __launch_bounds__(T1_BLOCK) void k_train_adjoint(const double *__rest
     T *rows = (T *)(used + NT);
     const int b = blockIdx.x;
     const double a = vel[b];
-    static_assert(OWN * NT >= MLP5_COUNT, "every parameter needs an owner");
-    int my_in[OWN], my_out[OWN];
-    double acc[OWN];
-    for (int o = 0; o < OWN; ++o) {
-        const int p = threadIdx.x + o * NT;
-        my_in[o] = my_out[o] = -1;
-        acc[o] = 0.0;
-        if (p < MLP5_COUNT) param_row_slots(p, my_in[o], my_out[o]);
-    }
+    static_assert(NT % 64 == 0 && NT / 64 * 128 == 2 * NT, "one 128-row MFMA pass per wave");
+    double acc[NTILE][4];
+    for (int t = 0; t < NTILE; ++t)
+        for (int q = 0; q < 4; ++q) acc[t][q] = 0.0;
     for (int j = threadIdx.x; j < n; j += NT) lam[j] = lam_T[(size_t)b * n + j];
     __syncthreads();
     for (int step = nsteps - 1; step >= 0; --step) {
@@ -181,7 +246,7 @@ __global__ __launch_bounds__(T1_BLOCK) void k_train_adjoint(const double *__rest
             g[j] = (2.0 / 3.0) * dt * lam[j];
         }
         __syncthreads();
-        vjp_rhs<SCHEME, NT>(u2, g, lx, adjq, rows, used, n, a, dxinv, P, my_in, my_out, acc);
+        vjp_rhs<SCHEME, NT>(u2, g, lx, adjq, rows, used, n, a, dxinv, P, acc);
         // stage 2: u2 = 3/4 u + 1/4 (u1 + dt R(u1))
         for (int j = threadIdx.x; j < n; j += NT) {
             lu[j] += 0.75 * lx[j];
@@ -189,20 +254,32 @@ __global__ __launch_bounds__(T1_BLOCK) void k_train_adjoint(const double *__rest
             lx[j] = 0.25 * lx[j];
         }
         __syncthreads();
-        vjp_rhs<SCHEME, NT>(u1, g, lx, adjq, rows, used, n, a, dxinv, P, my_in, my_out, acc);
+        vjp_rhs<SCHEME, NT>(u1, g, lx, adjq, rows, used, n, a, dxinv, P, acc);
         // stage 1: u1 = u + dt R(u)
         for (int j = threadIdx.x; j < n; j += NT) {
             lu[j] += lx[j];
             g[j] = dt * lx[j];
         }
         __syncthreads();
-        vjp_rhs<SCHEME, NT>(u, g, lu, adjq, rows, used, n, a, dxinv, P, my_in, my_out, acc);
+        vjp_rhs<SCHEME, NT>(u, g, lu, adjq, rows, used, n, a, dxinv, P, acc);
         for (int j = threadIdx.x; j < n; j += NT) lam[j] = lu[j];
         __syncthreads();
     }
-    for (int o = 0; o < OWN; ++o) {
-        const int p = threadIdx.x + o * NT;
-        if (p < MLP5_COUNT) grad[(size_t)b * MLP5_COUNT + p] = acc[o];
+    // sum the waves' tiles: red[wave][t][i][j] in the (now free) row buffer
+    double *red = (double *)rows;
+    {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int t = 0; t < NTILE; ++t)
+            for (int q = 0; q < 4; ++q)
+                red[((wave * NTILE + t) * 16 + Tile<T>::row_of(lane, q)) * 16 + (lane & 15)] = acc[t][q];
+    }
+    __syncthreads();
+    for (int p = threadIdx.x; p < MLP5_COUNT; p += NT) {
+        int t, i, j;
+        param_tile(p, t, i, j);
+        double sum = 0.0;
+        for (int w = 0; w < NT / 64; ++w) sum += red[((w * NTILE + t) * 16 + i) * 16 + j];
+        grad[(size_t)b * MLP5_COUNT + p] = sum;
     }
     if (lam0)
         for (int j = threadIdx.x; j < n; j += NT) lam0[(size_t)b * n + j] = lam[j];
