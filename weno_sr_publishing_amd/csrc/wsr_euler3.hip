@@ -16,10 +16,11 @@
 //                         component, position); the closure is evaluated once
 //                         for both Gauss points (-> S[first_lr][side]).
 //     k3_flux<d>        : second transverse Gauss pass fused with the Riemann
-//                         solves; two lanes per face (first_lr = 1 / 2), each
-//                         reconstructing both second_lr points, then the face
-//                         sum ((F1 + F2) + F3) + F4 in the reference's flag
-//                         order, times dt / h.
+//                         solves; four lanes per face for SR (first_lr x side),
+//                         each reconstructing its side at both second_lr points
+//                         and solving one of the four Riemann problems, two for
+//                         JS (first_lr), then the face sum ((F1 + F2) + F3) + F4
+//                         in the reference's flag order, times dt / h.
 //   k3_update           : q <- SSPRK3 combination of q0 and q - (dFx+dFy+dFz)/4;
 //                         in the device time loop the last stage also reduces
 //                         the next step's max speed.
@@ -52,12 +53,14 @@ constexpr int GC = 3;
 #define WSR3_WAVES_NORMAL 1
 #endif
 #ifndef WSR3_WAVES_FLUX
-#define WSR3_WAVES_FLUX 1
+#define WSR3_WAVES_FLUX 5
 #endif
 // waves per SIMD the register allocation of the normal / flux passes aims for
-// (the FP64 closure keeps the default)
+// (FP32-closure flux lanes fit 5; the others keep the compiler's default)
 template <int SCHEME> struct WavesNormal { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : WSR3_WAVES_NORMAL; };
-template <int SCHEME> struct WavesFlux { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : WSR3_WAVES_FLUX; };
+template <int SCHEME> struct WavesFlux {
+    static constexpr int value = (SCHEME == SCHEME_SR32 || SCHEME == SCHEME_SRBF16) ? WSR3_WAVES_FLUX : 1;
+};
 
 struct Grid3 {
     int nx, ny, nz, nxt, nyt, nzt;
@@ -445,20 +448,30 @@ __global__ __launch_bounds__(BLOCK) void k3_trans1(const double *__restrict__ NL
 }
 
 // second transverse pass + Riemann solves + flag sum (solver.py:176-203,
-// kernels.py:1016-1090); lanes 2f, 2f+1 own face f with first_lr = 1, 2
+// kernels.py:1016-1090).  With the SR closures four lanes own a face:
+// q = lane & 3 = 2 (first_lr - 1) + side (0: low side, from NR at the face's
+// index; 1: high side, from NL one cell up the normal axis); a lane reconstructs
+// its side's five components at both second_lr points, swaps one point with its
+// partner (lane ^ 1) and solves one Riemann problem (side 0: second_lr = 1,
+// side 1: second_lr = 2).  WENO-JS, whose reconstructions are cheap, uses two
+// lanes per face (first_lr), each reconstructing both sides and solving two.
+template <int SCHEME> struct FluxLanes { static constexpr int value = SCHEME == SCHEME_JS ? 2 : 4; };
+
 template <int DIR, int SCHEME, int RIEMANN>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WavesFlux<SCHEME>::value))) void k3_flux(const double *__restrict__ S, double *__restrict__ F,
-                                                 PassArgs<SCHEME> A, double dt_host) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WavesFlux<SCHEME>::value))) void k3_flux(
+    const double *__restrict__ S, double *__restrict__ F, PassArgs<SCHEME> A, double dt_host) {
     if (A.ctl && A.ctl->done) return;
+    constexpr int LPF = FluxLanes<SCHEME>::value;
     const Grid3 &g = A.g;
     constexpr int T2 = Trans<DIR>::T2;
     const int ex = DIR == 1 ? g.nx + 1 : g.nx, ey = DIR == 2 ? g.ny + 1 : g.ny, ez = DIR == 3 ? g.nz + 1 : g.nz;
     const long long nfaces = (long long)ex * ey * ez;
     long long lane = (long long)blockIdx.x * BLOCK + threadIdx.x;
-    long long face = lane >> 1;
-    const int a = (int)(lane & 1);             // first_lr - 1
+    long long face = lane / LPF;
+    const int q = (int)(lane % LPF);
+    const int a = LPF == 4 ? q >> 1 : q;       // first_lr - 1
     const bool active = face < nfaces;
-    if (!active) face = nfaces - 1;            // idle lanes still take part in the exchange
+    if (!active) face = nfaces - 1;            // idle lanes still take part in the exchanges
     const int i = (int)(face % ex);
     long long r = face / ex;
     const int j = (int)(r % ey), k = (int)(r / ey);
@@ -468,46 +481,85 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WavesFlux
     const double dt = A.ctl ? A.ctl->dt : dt_host;
     const double dt_h = dt / h;
     const long long plane = 5 * g.np;
-    // side 1 (from NR) at the face's own index is the low-side state, side 0
-    // (from NL) one cell up the normal axis is the high-side state
     const double *lo_src = S + (a * 2 + 1) * plane + pos;
     const double *hi_src = S + (a * 2 + 0) * plane + pos + sn;
-    V5 lo1 = {}, lo2 = {}, hi1 = {}, hi2 = {};
     LaneCounts cnt;
+    double tot[5];
+    const int base = threadIdx.x & ~(LPF - 1);
+    if constexpr (LPF == 4) {
+        const int side = q & 1;
+        const double *src = side == 0 ? lo_src : hi_src;
+        V5 v1 = {}, v2 = {};   // this side at second_lr = 1, 2
 #pragma unroll 1
-    for (int c = 0; c < 5; ++c) {
-        double w[5], g1, g2, e1, e2;
-        bool ev;
+        for (int c = 0; c < 5; ++c) {
+            double w[5], g1, g2;
+            bool ev;
 #pragma unroll
-        for (int m = 0; m < 5; ++m) w[m] = lo_src[c * g.np + (m + 1) * st];
-        recon_gauss<SCHEME>(w, A.mlp, A.eno, ht, g1, g2, ev);
-        if (A.counters) cnt.add(2, 5, active, ev);
+            for (int m = 0; m < 5; ++m) w[m] = src[c * g.np + (m + 1) * st];
+            recon_gauss<SCHEME>(w, A.mlp, A.eno, ht, g1, g2, ev);
+            if (A.counters) cnt.add(2, 5, active, ev);
 #pragma unroll
-        for (int m = 0; m < 5; ++m) w[m] = hi_src[c * g.np + (m + 1) * st];
-        recon_gauss<SCHEME>(w, A.mlp, A.eno, ht, e1, e2, ev);
-        if (A.counters) cnt.add(2, 5, active, ev);
+            for (int cc = 0; cc < 5; ++cc) {
+                if (cc == c) { v1.v[cc] = g1; v2.v[cc] = g2; }
+            }
+        }
+        // side 0 keeps point 1 and receives the high side's point 1; side 1 keeps
+        // point 2 and receives the low side's point 2
+        V5 lo, hi;
 #pragma unroll
-        for (int cc = 0; cc < 5; ++cc) {
-            if (cc == c) { lo1.v[cc] = g1; lo2.v[cc] = g2; hi1.v[cc] = e1; hi2.v[cc] = e2; }
+        for (int c = 0; c < 5; ++c) {
+            const double send = side == 0 ? v2.v[c] : v1.v[c];
+            const double got = __shfl_xor(send, 1);
+            lo.v[c] = side == 0 ? v1.v[c] : got;
+            hi.v[c] = side == 0 ? got : v2.v[c];
+        }
+        V5 fb;
+        if constexpr (RIEMANN == WSR_RIEMANN_EVILIN) fb = evilin_flux5<DIR>(lo, hi, dt_h, g.gamma);
+        else fb = hllc_flux5<DIR>(lo, hi, g.gamma);
+        // flags 1..4 = (first_lr, second_lr) (1,1), (2,1), (1,2), (2,2) live on q = 0, 2, 1, 3
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const double sv = fb.v[c] * dt_h;
+            const double f1 = __shfl(sv, base + 0), f2 = __shfl(sv, base + 2);
+            const double f3 = __shfl(sv, base + 1), f4 = __shfl(sv, base + 3);
+            tot[c] = ((f1 + f2) + f3) + f4;
+        }
+    } else {
+        V5 lo1 = {}, lo2 = {}, hi1 = {}, hi2 = {};
+#pragma unroll 1
+        for (int c = 0; c < 5; ++c) {
+            double w[5], g1, g2, e1, e2;
+            bool ev;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) w[m] = lo_src[c * g.np + (m + 1) * st];
+            recon_gauss<SCHEME>(w, A.mlp, A.eno, ht, g1, g2, ev);
+            if (A.counters) cnt.add(2, 5, active, ev);
+#pragma unroll
+            for (int m = 0; m < 5; ++m) w[m] = hi_src[c * g.np + (m + 1) * st];
+            recon_gauss<SCHEME>(w, A.mlp, A.eno, ht, e1, e2, ev);
+            if (A.counters) cnt.add(2, 5, active, ev);
+#pragma unroll
+            for (int cc = 0; cc < 5; ++cc) {
+                if (cc == c) { lo1.v[cc] = g1; lo2.v[cc] = g2; hi1.v[cc] = e1; hi2.v[cc] = e2; }
+            }
+        }
+        V5 fb1, fb2;   // second_lr = 1, 2
+        if constexpr (RIEMANN == WSR_RIEMANN_EVILIN) {
+            fb1 = evilin_flux5<DIR>(lo1, hi1, dt_h, g.gamma);
+            fb2 = evilin_flux5<DIR>(lo2, hi2, dt_h, g.gamma);
+        } else {
+            fb1 = hllc_flux5<DIR>(lo1, hi1, g.gamma);
+            fb2 = hllc_flux5<DIR>(lo2, hi2, g.gamma);
+        }
+        // flags 1..4: (a=0, 1), (a=1, 1), (a=0, 2), (a=1, 2)
+#pragma unroll
+        for (int c = 0; c < 5; ++c) {
+            const double s1 = fb1.v[c] * dt_h, s2 = fb2.v[c] * dt_h;      // this lane's (a, 1), (a, 2)
+            const double o1 = __shfl_xor(s1, 1), o2 = __shfl_xor(s2, 1);   // the partner's
+            tot[c] = ((s1 + o1) + s2) + o2;                                 // used on the a = 0 lane
         }
     }
-    V5 fb1, fb2;   // second_lr = 1, 2
-    if constexpr (RIEMANN == WSR_RIEMANN_EVILIN) {
-        fb1 = evilin_flux5<DIR>(lo1, hi1, dt_h, g.gamma);
-        fb2 = evilin_flux5<DIR>(lo2, hi2, dt_h, g.gamma);
-    } else {
-        fb1 = hllc_flux5<DIR>(lo1, hi1, g.gamma);
-        fb2 = hllc_flux5<DIR>(lo2, hi2, g.gamma);
-    }
-    // flags: 1 = (1,1), 2 = (2,1), 3 = (1,2), 4 = (2,2) as (first_lr, second_lr)
-    double tot[5];
-#pragma unroll
-    for (int c = 0; c < 5; ++c) {
-        double s1 = fb1.v[c] * dt_h, s2 = fb2.v[c] * dt_h;      // this lane's (a, 1), (a, 2)
-        double o1 = __shfl_xor(s1, 1), o2 = __shfl_xor(s2, 1);   // the partner's
-        tot[c] = ((s1 + o1) + s2) + o2;                           // used on the a = 0 lane
-    }
-    if (active && a == 0) {
+    if (active && q == 0) {
 #pragma unroll
         for (int c = 0; c < 5; ++c) F[c * g.np + pos] = tot[c];
     }
@@ -747,7 +799,7 @@ static void launch_direction(double *q, const Workspace3 &w, double *F, const Gr
     const long long nf = (long long)(DIR == 1 ? g.nx + 1 : g.nx) * (DIR == 2 ? g.ny + 1 : g.ny) *
                          (DIR == 3 ? g.nz + 1 : g.nz);
     if (tm) tm->begin(st);
-    hipLaunchKernelGGL((k3_flux<DIR, SCHEME, RIEMANN>), dim3(blocks_for(2 * nf)), dim3(BLOCK), 0, st,
+    hipLaunchKernelGGL((k3_flux<DIR, SCHEME, RIEMANN>), dim3(blocks_for(FluxLanes<SCHEME>::value * nf)), dim3(BLOCK), 0, st,
                        (const double *)w.S, F, A, dt);
     if (tm) tm->end(2, st);
 }
