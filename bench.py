@@ -55,7 +55,12 @@ BYTES_PER_SWEEP_CELL = 32 + 32               # state tile read + flux write (FP6
 # HBM bytes per SR32 sweep launch at 400^2 (mean of x and y), from the two
 # rocprofv3 counter passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000 after the
 # copy-kernel calibration) recorded in profiles/pmc_traffic_k_sweep_c4_400x400.md
-SWEEP_TRAFFIC_BYTES_400 = 21.99e6
+SWEEP_TRAFFIC_BYTES_400 = 22.10e6
+# the same for the SR32 3-D flux launch at 224x88x88 (mean of the x, y, z face
+# passes; profiles/pmc_traffic_k3_shockbubble_224x88x88.md) and the SR32 L+L~ WENO7
+# sweep at 1200x300 (mean of x and y; profiles/pmc_traffic_k7_doublemach_1200x300.md)
+FLUX3_TRAFFIC_BYTES_224 = 1257.5e6
+SWEEP7_TRAFFIC_BYTES_1200 = 68.4e6
 
 
 def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -> float:
@@ -245,7 +250,8 @@ def bench_shockbubble3d(args) -> int:
             "peak": PEAK_VALU_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_VALU_TFLOPS,
-            "traffic": None,
+            "traffic": FLUX3_TRAFFIC_BYTES_224 if args.closure == "sr32" else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": flux_ms,
             "launches": kn[2],
             "algorithmic_flops_per_launch": flops,
@@ -412,7 +418,8 @@ def bench_doublemach7(args) -> int:
             "peak": PEAK_VALU_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_VALU_TFLOPS,
-            "traffic": None,
+            "traffic": SWEEP7_TRAFFIC_BYTES_1200 if closure == "sr32" else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": both_ms,
             "launches": kn[0],
             "l_only_sweep": {"avg_launch_ms": kms[1] / max(1, kn[1]), "launches": kn[1]},
@@ -545,8 +552,9 @@ def bench_train1d(args) -> int:
         "loss_first_last": [losses[0], losses[-1]],
         "roofline": {
             "bound": "mfma",
-            "compute": "VALU (no MFMA): FP32 closure MLP forward/backward + FP64 state; peak is the MI355X "
-                       "FP64-vector / non-packed FP32-vector rate",
+            "compute": "VALU: FP32 closure MLP forward/backward + FP64 state; the parameter-gradient sums "
+                       "run as 16x16x4 MFMA outer products; peak is the MI355X FP64-vector / non-packed "
+                       "FP32-vector rate",
             "kernel": "k_train_adjoint<sr32>",
             "achieved": achieved,
             "peak": PEAK_VALU_TFLOPS,
@@ -568,18 +576,20 @@ def bench_train1d(args) -> int:
         oracle.set_threads(threads)
         done, t0 = 0, time.perf_counter()
         while True:
-            oracle.adv1d_run(u0[:32], vel[:32], dts, float(N), scheme="sr32",
+            lo = done % B
+            oracle.adv1d_run(u0[lo:lo + 32], vel[lo:lo + 32], dts, float(N), scheme="sr32",
                              mlp=trainer.mlp())
             done += 32
             el = time.perf_counter() - t0
-            if el >= args.cpu_seconds or done >= B:
+            if el >= args.cpu_seconds:
                 break
         cpu_rate = done * N * nsteps / el
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s (forward rollout only)", "cores": threads,
                                "kind": "port",
-                               "sample": f"{done} of the 512 trajectories rolled forward 64 cells ({nsteps} steps, "
-                                         f"sr32) by the OpenMP C oracle in {el:.1f} s; the oracle has no adjoint: "
-                                         "a gradient by central differences costs 2 x 189 + 1 such rollouts",
+                               "sample": f"{done} trajectories (the batch of {B}, cycled in chunks of 32) rolled "
+                                         f"forward 64 cells ({nsteps} steps, sr32) by the OpenMP C oracle in "
+                                         f"{el:.1f} s; the oracle has no adjoint: a gradient by central "
+                                         "differences costs 2 x 189 + 1 such rollouts",
                                "cpu_model": cpu_model_name(), "host_cpus": os.cpu_count()}
         out["gpu_vs_cpu"] = value / cpu_rate
     print(json.dumps(out), flush=True)
@@ -673,14 +683,19 @@ def bench_gste1d(args) -> int:
     if not args.no_cpu:
         from oracle import oracle
         oracle.set_threads(1)
-        t0 = time.perf_counter()
-        oracle.adv1d_run(u0[None, :], [1.0], dts, 1.0 / dx, scheme=closure, mlp=models[closure])
-        el = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": n * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+        runs, t0 = 0, time.perf_counter()
+        while True:
+            oracle.adv1d_run(u0[None, :], [1.0], dts, 1.0 / dx, scheme=closure, mlp=models[closure])
+            runs += 1
+            el = time.perf_counter() - t0
+            if el >= min(args.cpu_seconds, 5.0):
+                break
+        cpu_rate = runs * n * steps / el
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s", "cores": 1, "kind": "port",
                                "sample": f"the whole GSTE run ({steps} steps, {closure}) by the C oracle on one "
-                                         f"thread, {el:.2f} s", "cpu_model": cpu_model_name(),
-                               "host_cpus": os.cpu_count()}
-        out["gpu_vs_cpu"] = value / (n * steps / el)
+                                         f"thread, repeated {runs} times in {el:.2f} s",
+                               "cpu_model": cpu_model_name(), "host_cpus": os.cpu_count()}
+        out["gpu_vs_cpu"] = value / cpu_rate
     print(json.dumps(out), flush=True)
     return 0
 
