@@ -11,6 +11,8 @@ Sources (relative to the reference checkout):
   experiments/01_gste_long_time_advection/tables/gste_metrics.csv (1-D GSTE metrics)
   experiments/05_isentropic_vortex/tables/vortex_convergence.md (2-D vortex L1/L2, 9 digits)
   experiments/14_mixed_precision_timing/tables/weno5_precision_timing.tex (timings)
+  experiments/02_sod_shock_tube/tables/sod_metrics.csv          (Sod N=100 density errors, 7 methods)
+  experiments/03_lax_shock_tube/tables/lax_metrics.csv          (Lax N=200 primitive errors, 7 methods)
 """
 from __future__ import annotations
 
@@ -96,7 +98,16 @@ def timing() -> None:
         indent=2) + "\n")
 
 
+def shock_tubes() -> None:
+    out = {}
+    for key, rel in (("sod", "experiments/02_sod_shock_tube/tables/sod_metrics.csv"),
+                     ("lax", "experiments/03_lax_shock_tube/tables/lax_metrics.csv")):
+        out[key] = {"source": rel, "rows": list(csv.DictReader((REF / rel).open()))}
+    (OUT / "shock_tube_metrics.json").write_text(json.dumps(out, indent=2) + "\n")
+
+
 if __name__ == "__main__":
+    shock_tubes()
     smoke()
     mlp_golden()
     hllc()

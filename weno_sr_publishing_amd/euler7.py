@@ -67,6 +67,9 @@ def double_mach_params7(nx: int = 1200, ny: int = 300, cfl: float = 0.4, t_end: 
     return Params7(nx=nx, ny=ny, x_min=0.0, x_max=4.0, y_min=0.0, y_max=1.0, cfl=cfl, t_end=t_end)
 
 
+BOUNDARY7_ALIASES = {"outflow": "transmissive"}
+
+
 class Euler7Solver:
     def __init__(self, params: Params7, scheme: str = "js", riemann: str = "hllc",
                  boundary: str = "periodic", mlp: Mlp7Params | None = None, eno_cutoff: bool = False,
@@ -77,6 +80,9 @@ class Euler7Solver:
             raise ValueError(f"unknown WENO7 scheme {scheme!r}")
         if riemann not in _lib.RIEMANN:
             raise ValueError(f"unknown Riemann solver {riemann!r}")
+        # point_rk4.py:446-455 clamps the ghost cells for every name but "periodic";
+        # run_sod.py / run_quadrant.py call it "outflow"
+        boundary = BOUNDARY7_ALIASES.get(boundary, boundary)
         if boundary not in _lib.BOUNDARY:
             raise ValueError(f"unsupported boundary {boundary!r}")
         if loop not in _lib.LOOP7:
