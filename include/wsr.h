@@ -55,8 +55,12 @@ enum {
 /* instrumentation flags (wsr_euler2d_config.flags) */
 enum {
     WSR_FLAG_COUNT_CLOSURE = 1, /* count reconstructions and closure (MLP) evaluations into the workspace */
-    WSR_FLAG_Z_FALLBACK = 2     /* WENO7-SR: a closure ratio that is not finite or outside [0, 1] switches that
+    WSR_FLAG_Z_FALLBACK = 2,    /* WENO7-SR: a closure ratio that is not finite or outside [0, 1] switches that
                                    reconstruction to WENO7-Z (no reference counterpart) */
+    WSR_FLAG_Z7_UNIT_GAUSS_H = 4 /* WENO7-Z: transverse Gauss-point weights with h = 1 (epsilon = 240) instead of
+                                   the transverse spacing.  The reference's source passes the spacing
+                                   (weno_z_borges_p2_results/weno7_point_rk4_z.py:473-500); its recorded 2-D
+                                   WENO7-Z results (experiments/07-09 line cuts) follow h = 1. */
 };
 
 /* Grid and method description (a plain C struct; mirrors the fields of

@@ -13,6 +13,7 @@ Sources (relative to the reference checkout):
   experiments/14_mixed_precision_timing/tables/weno5_precision_timing.tex (timings)
   experiments/02_sod_shock_tube/tables/sod_metrics.csv          (Sod N=100 density errors, 7 methods)
   experiments/03_lax_shock_tube/tables/lax_metrics.csv          (Lax N=200 primitive errors, 7 methods)
+  experiments/06-09_riemann_c3..c6/tables/linecuts.csv          (C.3-C.6 density line cuts, 6 methods)
 """
 from __future__ import annotations
 
@@ -106,7 +107,17 @@ def shock_tubes() -> None:
     (OUT / "shock_tube_metrics.json").write_text(json.dumps(out, indent=2) + "\n")
 
 
+def linecuts() -> None:
+    out = {}
+    for case, folder in (("c3", "06_riemann_c3"), ("c4", "07_riemann_c4"), ("c5", "08_riemann_c5"),
+                         ("c6", "09_riemann_c6")):
+        rel = f"experiments/{folder}/tables/linecuts.csv"
+        out[case] = {"source": rel, "rows": list(csv.DictReader((REF / rel).open()))}
+    (OUT / "riemann_linecuts.json").write_text(json.dumps(out, indent=2) + "\n")
+
+
 if __name__ == "__main__":
+    linecuts()
     shock_tubes()
     smoke()
     mlp_golden()

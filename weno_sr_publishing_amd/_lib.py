@@ -28,6 +28,7 @@ EXPORTS = (
 )
 FLAG_COUNT_CLOSURE = 1
 FLAG_Z_FALLBACK = 2
+FLAG_Z7_UNIT_GAUSS_H = 4
 LOOP7 = {"point": 0, "double-mach": 1}
 
 

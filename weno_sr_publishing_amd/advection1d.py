@@ -38,6 +38,8 @@ class Adv1DSolver:
         self.device = torch.device(device)
         if self.device.type != "cuda":
             raise ValueError("the 1-D rollout runs on the GPU only")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         self._mlp_host = None
         if self.scheme != "js":
             dtype = np.float64 if self.scheme == "sr64" else np.float32

@@ -118,6 +118,7 @@ struct SweepArgs7 {
     unsigned long long *counters;
     int eno;
     int z_fallback;
+    int unit_gauss_h;   // WSR_FLAG_Z7_UNIT_GAUSS_H: transverse weights with h = 1
 };
 
 template <int DIR>
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Wa
     }
     const Grid &g = A.g;
     const double h_n = DIR == 1 ? g.dx : g.dy;
-    const double h_t = DIR == 1 ? g.dy : g.dx;
+    const double h_t = A.unit_gauss_h ? 1.0 : (DIR == 1 ? g.dy : g.dx);
     const double dt_h = dt / h_n;
     const int nfaces = (DIR == 1 ? g.nx : g.ny) + 1;
     const int ntrans = DIR == 1 ? g.ny : g.nx;
@@ -564,6 +565,7 @@ static void enqueue_rhs7(const wsr_euler7_config *cfg, const Workspace7 &w, cons
     SweepArgs7 A;
     A.g = g; A.dt_host = dt; A.ctl = ctl; A.eno = cfg->eno_cutoff;
     A.z_fallback = (cfg->flags & WSR_FLAG_Z_FALLBACK) ? 1 : 0;
+    A.unit_gauss_h = (cfg->flags & WSR_FLAG_Z7_UNIT_GAUSS_H) ? 1 : 0;
     A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
     if (tm) tm->begin(st);
     launch7<1>(cfg, w.bc, w.Fx, both ? w.Fxt : nullptr, A, w.mlp, st);

@@ -73,7 +73,8 @@ BOUNDARY7_ALIASES = {"outflow": "transmissive"}
 class Euler7Solver:
     def __init__(self, params: Params7, scheme: str = "js", riemann: str = "hllc",
                  boundary: str = "periodic", mlp: Mlp7Params | None = None, eno_cutoff: bool = False,
-                 loop: str = "point", z_fallback: bool = False, device: str | torch.device = "cuda"):
+                 loop: str = "point", z_fallback: bool = False, device: str | torch.device = "cuda",
+                 unit_gauss_h: bool = False):
         if params.ghost != 4:
             raise ValueError("the WENO7 step needs ghost = 4")
         if scheme not in ("js", "sr64", "sr32", "z"):
@@ -101,7 +102,7 @@ class Euler7Solver:
             if self._mlp_host.size != 908:
                 raise ValueError("WENO7-SR needs a 6-24-16-16-4 closure")
         self._lib = _lib.lib()
-        flags = _lib.FLAG_Z_FALLBACK if z_fallback else 0
+        flags = (_lib.FLAG_Z_FALLBACK if z_fallback else 0) | (_lib.FLAG_Z7_UNIT_GAUSS_H if unit_gauss_h else 0)
         self.cfg = _lib.Euler7Config(
             nx=params.nx, ny=params.ny, ghost=4, scheme=_lib.SCHEMES[scheme], riemann=_lib.RIEMANN[riemann],
             boundary=_lib.BOUNDARY[boundary], eno_cutoff=1 if eno_cutoff else 0, flags=flags,

@@ -47,12 +47,15 @@ def run_weno5(initial: np.ndarray, params: Params, *, device: str, boundary: str
 
 
 def run_weno7(initial: np.ndarray, params, *, device: str, boundary: str, report_interval: int = 0,
-              weight_kind: int = 1) -> tuple[np.ndarray, dict[str, Any]]:
+              weight_kind: int = 1, unit_gauss_h: bool = False) -> tuple[np.ndarray, dict[str, Any]]:
+    """``unit_gauss_h``: evaluate the transverse Gauss-point Z weights with h = 1 (epsilon = 240)
+    instead of the transverse spacing that weno7_point_rk4_z.py:473-500 passes; the
+    reference's recorded 2-D WENO7-Z results (the C.4-C.6 line-cut tables) follow h = 1."""
     if weight_kind != 1:
         raise ValueError("only the WENO7-Z weights of weight_kind = 1 (p = 3, epsilon = h^3) are built")
     from ..euler7 import Euler7Solver
     point_rk4._check("hllc", True)
-    solver = Euler7Solver(params, "z", "hllc", boundary, loop="point", device=device)
+    solver = Euler7Solver(params, "z", "hllc", boundary, loop="point", device=device, unit_gauss_h=unit_gauss_h)
     return point_rk4._run(solver, initial, params, report_interval, 10_000_000, "Z-RK4 ")
 
 
