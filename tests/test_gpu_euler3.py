@@ -166,3 +166,21 @@ def test_closure_counters3(mlp64, mlp32, gpu_device, scheme):
         assert c[1] == 0 and c[3] == 0
     else:
         assert 0 < c[1] <= c[0] and 0 < c[3] <= c[2]
+
+
+@pytest.mark.parametrize("scheme,steps", [("js", 823), ("sr64", 863), ("sr32", 863)])
+def test_shockbubble_ma3_step_counts_to_1e4(mlp64, mlp32, gpu_device, scheme, steps):
+    """The 224 x 88 x 88 Ma = 3 runs of the 3-D paper study (CFL 0.25, EVILIN, t = 1e-4)
+    take the step counts the reference recorded for them
+    (experiments/13_shock_bubble_3d_ma30/tables/density_line_x0085_errors.csv, "step"
+    column: WENO5-JS 823, WENO5-SR FP64 863, WENO5-SR FP32 863)."""
+    from dataclasses import replace
+    from weno_sr_publishing_amd.euler3d import Euler3Solver, ShockBubbleConfig
+    cfg = replace(ShockBubbleConfig(), t_end=1.0e-4)
+    mlp = None if scheme == "js" else (mlp64 if scheme == "sr64" else mlp32)
+    s = Euler3Solver(cfg, scheme=scheme, riemann="evilin", mlp=mlp)
+    s.initialize()
+    print(f"3-D Ma=3 {scheme} 224x88x88 to t=1e-4", flush=True)
+    dts, n, t = s.run()
+    assert s.diagnostics()["nan_count"] == 0
+    assert (n, t) == (steps, 1.0e-4)
