@@ -52,25 +52,28 @@ GAUSS = {"js": 101, "z": 106,
 FACE_LINE_FIXED = 68 + 42 + 40 * 8 + 8 * 7   # Roe + eigenvalues, left rows, 40 projections, back-projection
 FLUX_FACE_FIXED = 2 * 110 + 12               # two HLLC fluxes + dt/h scaling
 BYTES_PER_SWEEP_CELL = 32 + 32               # state tile read + flux write (FP64 x 4)
-# HBM bytes per SR32 sweep launch at 400^2 (mean of x and y), from the two
-# rocprofv3 counter passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000 after the
+# HBM bytes per SR32 sweep launch at 400^2 (one launch = the x and the y tiles), from
+# the two rocprofv3 counter passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000 after the
 # copy-kernel calibration) recorded in profiles/pmc_traffic_k_sweep_c4_400x400.md
-SWEEP_TRAFFIC_BYTES_400 = 22.10e6
+SWEEP_TRAFFIC_BYTES_400 = None
 # the same for the SR32 3-D flux launch at 224x88x88 (mean of the x, y, z face
 # passes; profiles/pmc_traffic_k3_shockbubble_224x88x88.md) and the SR32 L+L~ WENO7
-# sweep at 1200x300 (mean of x and y; profiles/pmc_traffic_k7_doublemach_1200x300.md)
+# sweep launch at 1200x300 (x and y tiles; profiles/pmc_traffic_k7_doublemach_1200x300.md)
 FLUX3_TRAFFIC_BYTES_224 = 1257.5e6
-SWEEP7_TRAFFIC_BYTES_1200 = 68.4e6
+SWEEP7_TRAFFIC_BYTES_1200 = None
 
 
 def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -> float:
-    nfaces, ntrans = NX + 1, NY
+    """One k_sweep_xy launch: the x tiles ((nx+1) faces x ny lines) and the y tiles."""
     if scheme.startswith("sr"):
         n = NORMAL["sr"][0] + frac_normal * (NORMAL["sr"][1] - NORMAL["sr"][0])
         g = GAUSS["sr"][0] + frac_gauss * (GAUSS["sr"][1] - GAUSS["sr"][0])
     else:
         n, g = NORMAL[scheme], GAUSS[scheme]
-    return nfaces * (ntrans + 4) * (FACE_LINE_FIXED + 8 * n) + nfaces * ntrans * (FLUX_FACE_FIXED + 8 * g)
+    total = 0.0
+    for nfaces, ntrans in ((NX + 1, NY), (NY + 1, NX)):
+        total += nfaces * (ntrans + 4) * (FACE_LINE_FIXED + 8 * n) + nfaces * ntrans * (FLUX_FACE_FIXED + 8 * g)
+    return total
 
 
 def copy_calibration(args, out: dict, torch, dev) -> None:
@@ -306,7 +309,7 @@ FACE_LINE7_FIXED = 68 + 8 * (10 + 56 + 7)  # Roe + eigen; per lane: left row, 7 
 
 
 def sweep7_flops_per_launch(scheme: str, frac_n: float, frac_g: float, both: bool) -> float:
-    """Mean of the x and y k7_sweep launches at 1200x300."""
+    """One k7_sweep_xy launch at 1200x300: the x tiles and the y tiles."""
     if scheme.startswith("sr"):
         n = NORMAL7["sr"][0] + frac_n * (NORMAL7["sr"][1] - NORMAL7["sr"][0])
         g = GAUSS7["sr"][0] + frac_g * (GAUSS7["sr"][1] - GAUSS7["sr"][0])
@@ -316,7 +319,7 @@ def sweep7_flops_per_launch(scheme: str, frac_n: float, frac_g: float, both: boo
     total = 0.0
     for nfaces, ntrans in ((NX7 + 1, NY7), (NY7 + 1, NX7)):
         total += nfaces * (ntrans + 6) * (FACE_LINE7_FIXED + 8 * n) + nfaces * ntrans * (8 * g + flux)
-    return total / 2.0
+    return total
 
 
 def bench_doublemach7(args) -> int:
@@ -384,7 +387,7 @@ def bench_doublemach7(args) -> int:
     js_achieved = sweep7_flops_per_launch("js", 0.0, 0.0, True) / (js_both_ms * 1e-3) / 1e12
     ncell = NX7 * NY7
     value = ncell * n_sr / wall_sr
-    hbm = ((NX7 + 8) * (NY7 + 8) * 32 + ((NX7 + 1) * NY7 + NX7 * (NY7 + 1)) / 2.0 * 64) / (both_ms * 1e-3) / 1e9
+    hbm = ((NX7 + 8) * (NY7 + 8) * 32 + ((NX7 + 1) * NY7 + NX7 * (NY7 + 1)) * 64) / (both_ms * 1e-3) / 1e9
     out = {
         "metric": "2D Euler cell-updates/s, WENO7-SR vs WENO7-JS, 1200x300 double Mach reflection",
         "value": value,
@@ -413,7 +416,7 @@ def bench_doublemach7(args) -> int:
             "bound": "mfma",
             "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
                        "FP64-vector / non-packed FP32-vector rate",
-            "kernel": f"k7_sweep<x|y,{closure},hllc,L+L~>",
+            "kernel": f"k7_sweep_xy<{closure},hllc,L+L~> (x and y tiles of a right-hand side in one launch)",
             "achieved": achieved,
             "peak": PEAK_VALU_TFLOPS,
             "unit": "TFLOP/s",
@@ -818,7 +821,7 @@ def main() -> int:
     achieved = flops / (avg_ms * 1e-3) / 1e12
     js_avg_ms = js_sweep_ms / max(1, js_sweep_n)
     js_achieved = sweep_flops_per_launch("js", 1.0, 1.0) / (js_avg_ms * 1e-3) / 1e12
-    hbm_gbs = (NX + 1) * (NY + 4) * BYTES_PER_SWEEP_CELL / (avg_ms * 1e-3) / 1e9
+    hbm_gbs = ((NX + 1) * (NY + 4) + (NY + 1) * (NX + 4)) * BYTES_PER_SWEEP_CELL / (avg_ms * 1e-3) / 1e9
 
     out = {
         "metric": METRIC,
@@ -849,7 +852,7 @@ def main() -> int:
             "bound": "mfma",
             "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
                        "FP64-vector / non-packed FP32-vector rate",
-            "kernel": f"k_sweep<x|y,{args.closure},hllc>",
+            "kernel": f"k_sweep_xy<{args.closure},hllc> (x and y tiles of a stage in one launch)",
             "achieved": achieved,
             "peak": PEAK_VALU_TFLOPS,
             "unit": "TFLOP/s",

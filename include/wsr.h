@@ -121,8 +121,8 @@ int wsr_euler2d_loop_status(void *workspace, int32_t *steps, double *t, double *
 
 /* Instrumented loop_steps (no reference counterpart; used by the benchmark):
  * the same n_steps (<= 64) steps with a HIP event pair around every sweep-kernel
- * launch on `stream`; synchronises and returns the summed sweep-kernel time of
- * the launches that ran and their number. */
+ * launch (one launch sweeps x and y of a stage) on `stream`; synchronises and
+ * returns the summed sweep-kernel time of the launches that ran and their number. */
 int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, double *u1, double *u2,
                                  double *dts, int32_t n_steps, void *workspace, size_t workspace_bytes,
                                  void *stream, double *sweep_ms, int32_t *sweep_launches);
@@ -229,10 +229,10 @@ int wsr_euler7_loop_steps(const wsr_euler7_config *cfg, double *u, double *dts, 
                           size_t workspace_bytes, void *stream);
 
 /* Instrumented loop_steps (no reference counterpart; used by the benchmark):
- * n_steps (<= 32) steps with a HIP event pair around every sweep launch on
- * `stream`; synchronises and returns the summed sweep time (ms) and launch
- * count of the sweeps that did work, [0] for the sweeps that serve L and L~,
- * [1] for the L-only sweeps. */
+ * n_steps (<= 32) steps with a HIP event pair around every sweep launch (one
+ * launch sweeps x and y of a right-hand side) on `stream`; synchronises and
+ * returns the summed sweep time (ms) and launch count of the sweeps that did
+ * work, [0] for the sweeps that serve L and L~, [1] for the L-only sweeps. */
 int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double *dts, int32_t n_steps,
                                 void *workspace, size_t workspace_bytes, void *stream, double *sweep_ms,
                                 int32_t *launches);

@@ -1,9 +1,10 @@
 // wsr_euler2d.hip -- MI355X (gfx950) kernels and C ABI of the 2-D WENO5 step.
 //
-// One SSPRK3 stage is four launches:
+// One SSPRK3 stage is three launches:
 //   k_boundary : ghost fill of the stage state into the workspace copy `bc`
 //                (transmissive / periodic / double-Mach).
-//   k_sweep<X> : fused x sweep.  Phase 1 computes, per x-face and row, the Roe
+//   k_sweep_xy : the x and the y sweep in one launch (blockIdx.z).  The x tiles:
+//                Phase 1 computes, per x-face and row, the Roe
 //                average, the characteristic projection of the six stencil
 //                cells, the two WENO5 face values (left and right state of the
 //                face) and their back-projection, into LDS.  Phase 2 computes,
@@ -11,7 +12,7 @@
 //                the eight face-value components (closure shared by both Gauss
 //                points) and the two Riemann fluxes, and writes
 //                Fx = (F(g1) + F(g2)) * dt/dx.
-//   k_sweep<Y> : the same along y -> Fy.
+//                The y tiles do the same along y -> Fy.
 //   k_update   : u_out = SSPRK3 combination of u0 and u - (dFx + dFy)/2; in the
 //                device time loop it also reduces the next step's max speed.
 // Each face is owned by eight lanes (side s = 0/1 x component k = 0..3); the
@@ -138,29 +139,27 @@ constexpr int REG_SIZE = P1_SIZE > P2_SIZE ? P1_SIZE : P2_SIZE;
 static_assert(N1 * SPLIT == SBLOCK, "phase 1a assigns one face-line per thread");
 static_assert(N1 % (SBLOCK / 8) == 0, "phase 1b passes cover the face-lines exactly");
 
-// Waves per SIMD the register allocation aims for: the tiles of a 400^2 sweep fit
-// on the chip in about one round, so latency hiding comes from resident waves;
-// the FP64 closure keeps the default (its MLP would spill below ~160 VGPRs).
+// Waves per SIMD the register allocation aims for: the sweep is latency-bound
+// (data-dependent closure branch, quarter-rate transcendentals, FP64 divisions), so
+// latency hiding comes from resident waves; the FP64 closure keeps the default (its
+// MLP would spill below ~160 VGPRs).
 template <int SCHEME> struct SweepWaves { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : 5; };
 
+// One 4 x 28 face tile of the DIR sweep (tile coordinates bx along the sweep axis, by
+// across it); smem is the block's LDS region of FV_SIZE + REG_SIZE doubles.
 template <int DIR, int SCHEME, int RIEMANN>
-__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWaves<SCHEME>::value))) void k_sweep(const double *__restrict__ bc, double *__restrict__ F,
-                                                 SweepArgs A, typename MlpSweep<SCHEME>::type mlp) {
-    double dt = A.dt_host;
-    if (A.ctl) {
-        if (A.ctl->done) return;
-        dt = A.ctl->dt;
-    }
+__device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double *__restrict__ F, const SweepArgs &A,
+                                           const typename MlpSweep<SCHEME>::type &mlp, double dt, double *smem,
+                                           int bx, int by) {
     const Grid &g = A.g;
     const double h_n = DIR == 1 ? g.dx : g.dy;   // sweep-axis spacing
     const double h_t = DIR == 1 ? g.dy : g.dx;   // transverse spacing
     const double dt_h = dt / h_n;
     const int nfaces = (DIR == 1 ? g.nx : g.ny) + 1;
     const int ntrans = DIR == 1 ? g.ny : g.nx;
-    const int f0 = blockIdx.x * TN;
-    const int t0 = blockIdx.y * TT;
+    const int f0 = bx * TN;
+    const int t0 = by * TT;
 
-    __shared__ double smem[FV_SIZE + REG_SIZE];
     double *FV = smem;
     double *T = smem + FV_SIZE;               // phase 1
     double *EIG = T + NN * NT1 * 4;
@@ -276,6 +275,28 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWav
         __syncthreads();
     }
     if (A.counters) cnt.flush(A.counters);
+}
+
+// Both sweeps of a stage in one launch (blockIdx.z = 0: x faces, 1: y faces).  They
+// read the same boundary-filled state and write separate flux arrays; one grid of
+// both tile sets keeps the CUs busy through the tail of the x tiles.
+template <int SCHEME, int RIEMANN>
+__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWaves<SCHEME>::value))) void k_sweep_xy(const double *__restrict__ bc, double *__restrict__ Fx,
+                                                    double *__restrict__ Fy, SweepArgs A,
+                                                    typename MlpSweep<SCHEME>::type mlp) {
+    double dt = A.dt_host;
+    if (A.ctl) {
+        if (A.ctl->done) return;
+        dt = A.ctl->dt;
+    }
+    __shared__ double smem[FV_SIZE + REG_SIZE];
+    if (blockIdx.z == 0) {
+        if ((int)blockIdx.x * TN > A.g.nx || (int)blockIdx.y * TT >= A.g.ny) return;
+        sweep_tile<1, SCHEME, RIEMANN>(bc, Fx, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+    } else {
+        if ((int)blockIdx.x * TN > A.g.ny || (int)blockIdx.y * TT >= A.g.nx) return;
+        sweep_tile<2, SCHEME, RIEMANN>(bc, Fy, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+    }
 }
 
 // ------------------------------------------------------------------ update
@@ -438,33 +459,33 @@ static void enqueue_params(const wsr_euler2d_config *cfg, const Workspace &w, hi
     hipLaunchKernelGGL(k_put_params, dim3(1), dim3(256), 0, st, pw, (uint32_t *)w.mlp, nwords);
 }
 
-template <int DIR, int SCHEME, int RIEMANN>
-static void launch_sweep_t(const double *bc, double *F, const SweepArgs &A, const void *mlp_dev,
-                           hipStream_t st) {
-    typename MlpSweep<SCHEME>::type mlp;
-    memset(&mlp, 0, sizeof mlp);
-    if constexpr (SCHEME == SCHEME_SR64) mlp.p = (const double *)mlp_dev;
-    else if constexpr (SCHEME == SCHEME_SR32 || SCHEME == SCHEME_SRBF16) mlp.p = (const float *)mlp_dev;
-    const int nfaces = (DIR == 1 ? A.g.nx : A.g.ny) + 1;
-    const int ntrans = DIR == 1 ? A.g.ny : A.g.nx;
-    dim3 grid((nfaces + TN - 1) / TN, (ntrans + TT - 1) / TT);
-    hipLaunchKernelGGL((k_sweep<DIR, SCHEME, RIEMANN>), grid, dim3(SBLOCK), 0, st, bc, F, A, mlp);
-}
-
-// optional per-launch timing of the sweep kernels (HIP events on the launch stream)
+// optional per-launch timing of the sweep kernel (HIP events on the launch stream)
 struct SweepTimer {
     hipEvent_t ev[2 * 6 * 64];
     int used = 0, cap = 0;
 };
 
-template <int DIR>
-static void launch_sweep(int scheme, int riemann, const double *bc, double *F, const SweepArgs &A,
-                         const void *mlp_dev, hipStream_t st, SweepTimer *timer = nullptr) {
+template <int SCHEME, int RIEMANN>
+static void launch_sweep_xy_t(const double *bc, double *Fx, double *Fy, const SweepArgs &A, const void *mlp_dev,
+                              hipStream_t st) {
+    typename MlpSweep<SCHEME>::type mlp;
+    memset(&mlp, 0, sizeof mlp);
+    if constexpr (SCHEME == SCHEME_SR64) mlp.p = (const double *)mlp_dev;
+    else if constexpr (SCHEME == SCHEME_SR32 || SCHEME == SCHEME_SRBF16) mlp.p = (const float *)mlp_dev;
+    const int gx = (A.g.nx + 1 + TN - 1) / TN, gy = (A.g.ny + 1 + TN - 1) / TN;
+    const int tx = (A.g.ny + TT - 1) / TT, ty = (A.g.nx + TT - 1) / TT;
+    dim3 grid(gx > gy ? gx : gy, tx > ty ? tx : ty, 2);
+    hipLaunchKernelGGL((k_sweep_xy<SCHEME, RIEMANN>), grid, dim3(SBLOCK), 0, st, bc, Fx, Fy, A, mlp);
+}
+
+// x and y sweeps of one stage in one launch (one timer event pair around it)
+static void launch_sweep_xy(int scheme, int riemann, const double *bc, double *Fx, double *Fy, const SweepArgs &A,
+                            const void *mlp_dev, hipStream_t st, SweepTimer *timer = nullptr) {
     if (timer && timer->used + 2 <= timer->cap) (void)hipEventRecord(timer->ev[timer->used], st);
-#define WSR_CASE(S)                                                                              \
-    case S:                                                                                      \
-        if (riemann == RIEMANN_FORCE) launch_sweep_t<DIR, S, RIEMANN_FORCE>(bc, F, A, mlp_dev, st); \
-        else launch_sweep_t<DIR, S, RIEMANN_HLLC>(bc, F, A, mlp_dev, st);                         \
+#define WSR_CASE(S)                                                                                    \
+    case S:                                                                                            \
+        if (riemann == RIEMANN_FORCE) launch_sweep_xy_t<S, RIEMANN_FORCE>(bc, Fx, Fy, A, mlp_dev, st); \
+        else launch_sweep_xy_t<S, RIEMANN_HLLC>(bc, Fx, Fy, A, mlp_dev, st);                           \
         break;
     switch (scheme) {
         WSR_CASE(SCHEME_JS)
@@ -492,8 +513,7 @@ static void enqueue_stage(const wsr_euler2d_config *cfg, const Workspace &w, con
     SweepArgs A;
     A.g = g; A.dt_host = dt; A.ctl = ctl; A.eno = cfg->eno_cutoff;
     A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
-    launch_sweep<1>(cfg->scheme, cfg->riemann, w.bc, w.Fx, A, w.mlp, st, timer);
-    launch_sweep<2>(cfg->scheme, cfg->riemann, w.bc, w.Fy, A, w.mlp, st, timer);
+    launch_sweep_xy(cfg->scheme, cfg->riemann, w.bc, w.Fx, w.Fy, A, w.mlp, st, timer);
     const int ncells = g.nx * g.ny;
     hipLaunchKernelGGL(k_update, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const double *)w.bc, u0,
                        u_out, (const double *)w.Fx, (const double *)w.Fy, g, rk, ctl, fold_speed, dts);
@@ -594,7 +614,7 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
     hipStream_t st = (hipStream_t)stream;
     SweepTimer timer;
-    timer.cap = 6 * 2 * n_steps;
+    timer.cap = 3 * 2 * n_steps;
     for (int i = 0; i < timer.cap; ++i)
         if (hipEventCreate(&timer.ev[i]) != hipSuccess) return WSR_E_HIP;
     if (n_steps > 0) enqueue_params(cfg, w, st);

@@ -1,8 +1,8 @@
 // wsr_euler7.hip -- MI355X (gfx950) kernels and C ABI of the 2-D WENO7 step
 // with Shu's fourth-order TVD Runge-Kutta method (downwind operator L~).
 //
-// One right-hand-side evaluation is a boundary fill (ghost 4) and two fused
-// sweeps.  A sweep tile computes, per face and transverse line, the Roe
+// One right-hand-side evaluation is a boundary fill (ghost 4) and one sweep
+// launch that covers the x and the y tiles.  A sweep tile computes, per face and transverse line, the Roe
 // average, the characteristic projection of the 8-cell window, both WENO7 face
 // values (lr 1 / lr 2) and their back-projection into LDS (phase 1), then the
 // 7-point transverse Gauss reconstruction of the eight face-value components
@@ -12,7 +12,7 @@
 // from the same face values: a Shu RK4 step costs four reconstructions
 // (L,L~ of u; L,L~ of u1; L of u2; L of u3) instead of six.
 //
-// A step is: per stage [boundary, x sweep, y sweep, stage combine]; the stage
+// A step is: per stage [boundary, x+y sweep, stage combine]; the stage
 // combines accumulate the RK4 sums in the reference's evaluation order.
 //
 // Reference (paths relative to the reference checkout):
@@ -137,26 +137,23 @@ __device__ __forceinline__ void riemann7(const double ul[4], const double ur[4],
 
 // F[face] = F(g1) + F(g2) with the states (left, right); with BOTH also
 // Ft[face] = F(right, left) at both points (the downwind operator).
+// One 4 x 26 face tile of the DIR sweep (tile coordinates bx along the sweep axis, by
+// across it); smem is the block's LDS region of FV_SIZE + REG_SIZE doubles.
 template <int DIR, int SCHEME, int RIEMANN, bool BOTH>
-__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Waves<SCHEME>::value))) void k7_sweep(const double *__restrict__ bc, double *__restrict__ F,
-                                                   double *__restrict__ Ft, SweepArgs7 A,
-                                                   typename Mlp7Sweep<SCHEME>::type mlp) {
-    double dt = A.dt_host;
-    if (A.ctl) {
-        if (A.ctl->done) return;
-        dt = A.ctl->dt;
-    }
+__device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, double *__restrict__ F,
+                                            double *__restrict__ Ft, const SweepArgs7 &A,
+                                            const typename Mlp7Sweep<SCHEME>::type &mlp, double dt, double *smem,
+                                            int bx, int by) {
     const Grid &g = A.g;
     const double h_n = DIR == 1 ? g.dx : g.dy;
     const double h_t = A.unit_gauss_h ? 1.0 : (DIR == 1 ? g.dy : g.dx);
     const double dt_h = dt / h_n;
     const int nfaces = (DIR == 1 ? g.nx : g.ny) + 1;
     const int ntrans = DIR == 1 ? g.ny : g.nx;
-    const int f0 = blockIdx.x * TN;
-    const int t0 = blockIdx.y * TT;
+    const int f0 = bx * TN;
+    const int t0 = by * TT;
     const bool zf = A.z_fallback != 0;
 
-    __shared__ double smem[FV_SIZE + REG_SIZE];
     double *FV = smem;
     double *T = smem + FV_SIZE;
     double *EIG = T + NN * NT1 * 4;
@@ -274,6 +271,28 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Wa
         __syncthreads();
     }
     if (A.counters) cnt.flush(A.counters);
+}
+
+// Both sweeps of a right-hand side in one launch (blockIdx.z = 0: x faces, 1: y faces);
+// they read the same boundary-filled state and write separate flux arrays.
+template <int SCHEME, int RIEMANN, bool BOTH>
+__global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Waves<SCHEME>::value))) void k7_sweep_xy(const double *__restrict__ bc, double *__restrict__ Fx,
+                                                      double *__restrict__ Fxt, double *__restrict__ Fy,
+                                                      double *__restrict__ Fyt, SweepArgs7 A,
+                                                      typename Mlp7Sweep<SCHEME>::type mlp) {
+    double dt = A.dt_host;
+    if (A.ctl) {
+        if (A.ctl->done) return;
+        dt = A.ctl->dt;
+    }
+    __shared__ double smem[FV_SIZE + REG_SIZE];
+    if (blockIdx.z == 0) {
+        if ((int)blockIdx.x * TN > A.g.nx || (int)blockIdx.y * TT >= A.g.ny) return;
+        sweep7_tile<1, SCHEME, RIEMANN, BOTH>(bc, Fx, Fxt, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+    } else {
+        if ((int)blockIdx.x * TN > A.g.ny || (int)blockIdx.y * TT >= A.g.nx) return;
+        sweep7_tile<2, SCHEME, RIEMANN, BOTH>(bc, Fy, Fyt, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+    }
 }
 
 // ------------------------------------------------------------------ stages
@@ -503,30 +522,33 @@ static void enqueue_params7(const wsr_euler7_config *cfg, const Workspace7 &w, h
     }
 }
 
-template <int DIR, int SCHEME, int RIEMANN>
-static void launch7_t(const double *bc, double *F, double *Ft, const SweepArgs7 &A, const void *mlp_dev,
-                      hipStream_t st) {
+template <int SCHEME, int RIEMANN>
+static void launch7_t(const double *bc, double *Fx, double *Fxt, double *Fy, double *Fyt, const SweepArgs7 &A,
+                      const void *mlp_dev, hipStream_t st) {
     typename Mlp7Sweep<SCHEME>::type mlp;
     memset(&mlp, 0, sizeof mlp);
     if constexpr (SCHEME == SCHEME_SR64) mlp.p = (const double *)mlp_dev;
     else if constexpr (SCHEME == SCHEME_SR32) mlp.p = (const float *)mlp_dev;
-    const int nfaces = (DIR == 1 ? A.g.nx : A.g.ny) + 1;
-    const int ntrans = DIR == 1 ? A.g.ny : A.g.nx;
-    dim3 grid((nfaces + TN - 1) / TN, (ntrans + TT - 1) / TT);
-    if (Ft)
-        hipLaunchKernelGGL((k7_sweep<DIR, SCHEME, RIEMANN, true>), grid, dim3(SBLOCK), 0, st, bc, F, Ft, A, mlp);
+    const int gx = (A.g.nx + 1 + TN - 1) / TN, gy = (A.g.ny + 1 + TN - 1) / TN;
+    const int tx = (A.g.ny + TT - 1) / TT, ty = (A.g.nx + TT - 1) / TT;
+    dim3 grid(gx > gy ? gx : gy, tx > ty ? tx : ty, 2);
+    if (Fxt)
+        hipLaunchKernelGGL((k7_sweep_xy<SCHEME, RIEMANN, true>), grid, dim3(SBLOCK), 0, st, bc, Fx, Fxt, Fy, Fyt, A,
+                           mlp);
     else
-        hipLaunchKernelGGL((k7_sweep<DIR, SCHEME, RIEMANN, false>), grid, dim3(SBLOCK), 0, st, bc, F, Ft, A, mlp);
+        hipLaunchKernelGGL((k7_sweep_xy<SCHEME, RIEMANN, false>), grid, dim3(SBLOCK), 0, st, bc, Fx, Fxt, Fy, Fyt, A,
+                           mlp);
 }
 
-template <int DIR>
-static void launch7(const wsr_euler7_config *cfg, const double *bc, double *F, double *Ft, const SweepArgs7 &A,
-                    const void *mlp_dev, hipStream_t st) {
-#define WSR7_CASE(S)                                                                                      \
-    case S:                                                                                               \
-        if (cfg->riemann == WSR_RIEMANN_FORCE) launch7_t<DIR, S, WSR_RIEMANN_FORCE>(bc, F, Ft, A, mlp_dev, st); \
-        else if (cfg->riemann == WSR_RIEMANN_EVILIN) launch7_t<DIR, S, WSR_RIEMANN_EVILIN>(bc, F, Ft, A, mlp_dev, st); \
-        else launch7_t<DIR, S, WSR_RIEMANN_HLLC>(bc, F, Ft, A, mlp_dev, st);                              \
+// x and y sweeps of one right-hand side (with BOTH: also the downwind fluxes Fxt, Fyt)
+static void launch7(const wsr_euler7_config *cfg, const double *bc, double *Fx, double *Fxt, double *Fy, double *Fyt,
+                    const SweepArgs7 &A, const void *mlp_dev, hipStream_t st) {
+#define WSR7_CASE(S)                                                                                               \
+    case S:                                                                                                        \
+        if (cfg->riemann == WSR_RIEMANN_FORCE) launch7_t<S, WSR_RIEMANN_FORCE>(bc, Fx, Fxt, Fy, Fyt, A, mlp_dev, st); \
+        else if (cfg->riemann == WSR_RIEMANN_EVILIN)                                                               \
+            launch7_t<S, WSR_RIEMANN_EVILIN>(bc, Fx, Fxt, Fy, Fyt, A, mlp_dev, st);                               \
+        else launch7_t<S, WSR_RIEMANN_HLLC>(bc, Fx, Fxt, Fy, Fyt, A, mlp_dev, st);                                \
         break;
     switch (cfg->scheme) {
         WSR7_CASE(SCHEME_JS)
@@ -568,10 +590,7 @@ static void enqueue_rhs7(const wsr_euler7_config *cfg, const Workspace7 &w, cons
     A.unit_gauss_h = (cfg->flags & WSR_FLAG_Z7_UNIT_GAUSS_H) ? 1 : 0;
     A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
     if (tm) tm->begin(st);
-    launch7<1>(cfg, w.bc, w.Fx, both ? w.Fxt : nullptr, A, w.mlp, st);
-    if (tm) tm->end(both ? 0 : 1, st);
-    if (tm) tm->begin(st);
-    launch7<2>(cfg, w.bc, w.Fy, both ? w.Fyt : nullptr, A, w.mlp, st);
+    launch7(cfg, w.bc, w.Fx, both ? w.Fxt : nullptr, w.Fy, both ? w.Fyt : nullptr, A, w.mlp, st);
     if (tm) tm->end(both ? 0 : 1, st);
 }
 
