@@ -122,7 +122,9 @@ int wsr_euler2d_loop_status(void *workspace, int32_t *steps, double *t, double *
 /* Instrumented loop_steps (no reference counterpart; used by the benchmark):
  * the same n_steps (<= 64) steps with a HIP event pair around every sweep-kernel
  * launch (one launch sweeps x and y of a stage) on `stream`; synchronises and
- * returns the summed sweep-kernel time of the launches that ran and their number. */
+ * returns the summed sweep-kernel time and number of the launches of the steps
+ * the loop ran (read from the control block before and after; launches enqueued
+ * after the loop stopped are not counted). */
 int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, double *u1, double *u2,
                                  double *dts, int32_t n_steps, void *workspace, size_t workspace_bytes,
                                  void *stream, double *sweep_ms, int32_t *sweep_launches);
@@ -231,8 +233,8 @@ int wsr_euler7_loop_steps(const wsr_euler7_config *cfg, double *u, double *dts, 
 /* Instrumented loop_steps (no reference counterpart; used by the benchmark):
  * n_steps (<= 32) steps with a HIP event pair around every sweep launch (one
  * launch sweeps x and y of a right-hand side) on `stream`; synchronises and
- * returns the summed sweep time (ms) and launch count of the sweeps that did
- * work, [0] for the sweeps that serve L and L~, [1] for the L-only sweeps. */
+ * returns the summed sweep time (ms) and launch count of the sweeps of the steps
+ * the loop ran, [0] for the sweeps that serve L and L~, [1] for the L-only sweeps. */
 int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double *dts, int32_t n_steps,
                                 void *workspace, size_t workspace_bytes, void *stream, double *sweep_ms,
                                 int32_t *launches);
@@ -282,8 +284,8 @@ int wsr_euler3_loop_steps(const wsr_euler3_config *cfg, double *q, double *dts, 
 /* Instrumented loop_steps (no reference counterpart; used by the benchmark):
  * the same n_steps (<= 32) steps with a HIP event pair around every normal,
  * first-transverse-pass and flux launch on `stream`; synchronises and returns
- * the summed time (ms) and the number of launches that did work for each of
- * the three kernel kinds (kernel_ms[3], launches[3]).  With
+ * the summed time (ms) and the number of launches of the steps the loop ran
+ * for each of the three kernel kinds (kernel_ms[3], launches[3]).  With
  * WSR_FLAG_COUNT_CLOSURE in cfg->flags the closure counters of
  * wsr_euler2d_counters accumulate over the 3-D passes as well. */
 int wsr_euler3_loop_steps_timed(const wsr_euler3_config *cfg, double *q, double *dts, int32_t n_steps,

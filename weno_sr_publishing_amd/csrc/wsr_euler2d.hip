@@ -3,7 +3,7 @@
 // One SSPRK3 stage is three launches:
 //   k_boundary : ghost fill of the stage state into the workspace copy `bc`
 //                (transmissive / periodic / double-Mach).
-//   k_sweep_xy : the x and the y sweep in one launch (blockIdx.z).  The x tiles:
+//   k_sweep_xy : the x and the y sweep in one launch.  The x tiles:
 //                Phase 1 computes, per x-face and row, the Roe
 //                average, the characteristic projection of the six stencil
 //                cells, the two WENO5 face values (left and right state of the
@@ -143,7 +143,10 @@ static_assert(N1 % (SBLOCK / 8) == 0, "phase 1b passes cover the face-lines exac
 // (data-dependent closure branch, quarter-rate transcendentals, FP64 divisions), so
 // latency hiding comes from resident waves; the FP64 closure keeps the default (its
 // MLP would spill below ~160 VGPRs).
-template <int SCHEME> struct SweepWaves { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : 5; };
+#ifndef WSR_SWEEP_WAVES
+#define WSR_SWEEP_WAVES 5
+#endif
+template <int SCHEME> struct SweepWaves { static constexpr int value = SCHEME == SCHEME_SR64 ? 1 : WSR_SWEEP_WAVES; };
 
 // One 4 x 28 face tile of the DIR sweep (tile coordinates bx along the sweep axis, by
 // across it); smem is the block's LDS region of FV_SIZE + REG_SIZE doubles.
@@ -277,9 +280,10 @@ __device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double
     if (A.counters) cnt.flush(A.counters);
 }
 
-// Both sweeps of a stage in one launch (blockIdx.z = 0: x faces, 1: y faces).  They
-// read the same boundary-filled state and write separate flux arrays; one grid of
-// both tile sets keeps the CUs busy through the tail of the x tiles.
+// Both sweeps of a stage in one launch: workgroups [0, gx*tx) take the x tiles
+// (gx along x, tx across), the rest the y tiles.  They read the same
+// boundary-filled state and write separate flux arrays; one grid of both tile sets
+// keeps the CUs busy through the tail of the x tiles.
 template <int SCHEME, int RIEMANN>
 __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWaves<SCHEME>::value))) void k_sweep_xy(const double *__restrict__ bc, double *__restrict__ Fx,
                                                     double *__restrict__ Fy, SweepArgs A,
@@ -290,12 +294,14 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWav
         dt = A.ctl->dt;
     }
     __shared__ double smem[FV_SIZE + REG_SIZE];
-    if (blockIdx.z == 0) {
-        if ((int)blockIdx.x * TN > A.g.nx || (int)blockIdx.y * TT >= A.g.ny) return;
-        sweep_tile<1, SCHEME, RIEMANN>(bc, Fx, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+    const int gx = (A.g.nx + 1 + TN - 1) / TN, tx = (A.g.ny + TT - 1) / TT;
+    const int gy = (A.g.ny + 1 + TN - 1) / TN;
+    const int id = blockIdx.x;
+    if (id < gx * tx) {
+        sweep_tile<1, SCHEME, RIEMANN>(bc, Fx, A, mlp, dt, smem, id % gx, id / gx);
     } else {
-        if ((int)blockIdx.x * TN > A.g.ny || (int)blockIdx.y * TT >= A.g.nx) return;
-        sweep_tile<2, SCHEME, RIEMANN>(bc, Fy, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+        const int iy = id - gx * tx;
+        sweep_tile<2, SCHEME, RIEMANN>(bc, Fy, A, mlp, dt, smem, iy % gy, iy / gy);
     }
 }
 
@@ -474,8 +480,8 @@ static void launch_sweep_xy_t(const double *bc, double *Fx, double *Fy, const Sw
     else if constexpr (SCHEME == SCHEME_SR32 || SCHEME == SCHEME_SRBF16) mlp.p = (const float *)mlp_dev;
     const int gx = (A.g.nx + 1 + TN - 1) / TN, gy = (A.g.ny + 1 + TN - 1) / TN;
     const int tx = (A.g.ny + TT - 1) / TT, ty = (A.g.nx + TT - 1) / TT;
-    dim3 grid(gx > gy ? gx : gy, tx > ty ? tx : ty, 2);
-    hipLaunchKernelGGL((k_sweep_xy<SCHEME, RIEMANN>), grid, dim3(SBLOCK), 0, st, bc, Fx, Fy, A, mlp);
+    hipLaunchKernelGGL((k_sweep_xy<SCHEME, RIEMANN>), dim3(gx * tx + gy * ty), dim3(SBLOCK), 0, st, bc, Fx, Fy, A,
+                       mlp);
 }
 
 // x and y sweeps of one stage in one launch (one timer event pair around it)
@@ -617,6 +623,8 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
     timer.cap = 3 * 2 * n_steps;
     for (int i = 0; i < timer.cap; ++i)
         if (hipEventCreate(&timer.ev[i]) != hipSuccess) return WSR_E_HIP;
+    int32_t step0 = 0, step1 = 0;
+    if (!loop_step_count(w.ctl, st, &step0)) rc = WSR_E_HIP;
     if (n_steps > 0) enqueue_params(cfg, w, st);
     for (int n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
@@ -624,16 +632,16 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
         enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st, &timer);
         enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st, &timer);
     }
-    rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess) rc = WSR_E_HIP;
-    // launches after the loop stopped return at once; count only the ones that did work
-    Ctl host;
-    if (hipMemcpy(&host, w.ctl, sizeof(Ctl), hipMemcpyDeviceToHost) != hipSuccess) rc = WSR_E_HIP;
+    if (rc == WSR_OK) rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+    if (!loop_step_count(w.ctl, st, &step1)) rc = WSR_E_HIP;
+    // only the launches of the steps that ran: one sweep launch per stage, in order
+    int ran = step1 - step0;
+    ran = ran < 0 ? 0 : (ran > n_steps ? n_steps : ran);
     double total = 0.0;
     int launches = 0;
-    for (int i = 0; i + 1 < timer.used; i += 2) {
+    for (int i = 0; i + 1 < timer.used && i < 2 * 3 * ran; i += 2) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, timer.ev[i], timer.ev[i + 1]) == hipSuccess && ms > 0.002f) {
+        if (hipEventElapsedTime(&ms, timer.ev[i], timer.ev[i + 1]) == hipSuccess) {
             total += ms;
             launches += 1;
         }

@@ -940,6 +940,8 @@ int wsr_euler3_loop_steps_timed(const wsr_euler3_config *cfg, double *q, double 
     tm->cap = 27 * n_steps;
     for (int i = 0; i < 2 * tm->cap; ++i)
         if (hipEventCreate(&tm->ev[i]) != hipSuccess) { delete tm; return WSR_E_HIP; }
+    int32_t step0 = 0, step1 = 0;
+    if (!loop_step_count(w.ctl, st, &step0)) rc = WSR_E_HIP;
     if (n_steps > 0) enqueue_params3(cfg, w, st);
     for (int n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(k3_dt, dim3(1), dim3(1), 0, st, w.ctl);
@@ -947,13 +949,15 @@ int wsr_euler3_loop_steps_timed(const wsr_euler3_config *cfg, double *q, double 
         enqueue_stage3(cfg, w, q, 2, 0.0, w.ctl, 0, dts, st, tm);
         enqueue_stage3(cfg, w, q, 3, 0.0, w.ctl, 1, dts, st, tm);
     }
-    rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess) rc = WSR_E_HIP;
+    if (rc == WSR_OK) rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+    if (!loop_step_count(w.ctl, st, &step1)) rc = WSR_E_HIP;
+    // only the launches of the steps that ran: 27 timed launches per step, in order
+    int ran = step1 - step0;
+    ran = ran < 0 ? 0 : (ran > n_steps ? n_steps : ran);
     for (int k = 0; k < 3; ++k) { kernel_ms[k] = 0.0; launches[k] = 0; }
-    for (int i = 0; i < tm->used; ++i) {
+    for (int i = 0; i < tm->used && i < 27 * ran; ++i) {
         float ms = 0.0f;
-        // launches after the loop stopped return at once; count only the ones that did work
-        if (hipEventElapsedTime(&ms, tm->ev[2 * i], tm->ev[2 * i + 1]) == hipSuccess && ms > 0.02f) {
+        if (hipEventElapsedTime(&ms, tm->ev[2 * i], tm->ev[2 * i + 1]) == hipSuccess) {
             kernel_ms[tm->kind[i]] += ms;
             launches[tm->kind[i]] += 1;
         }

@@ -273,8 +273,9 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
     if (A.counters) cnt.flush(A.counters);
 }
 
-// Both sweeps of a right-hand side in one launch (blockIdx.z = 0: x faces, 1: y faces);
-// they read the same boundary-filled state and write separate flux arrays.
+// Both sweeps of a right-hand side in one launch: workgroups [0, gx*tx) take the x
+// tiles (gx along x, tx across), the rest the y tiles; they read the same
+// boundary-filled state and write separate flux arrays.
 template <int SCHEME, int RIEMANN, bool BOTH>
 __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Waves<SCHEME>::value))) void k7_sweep_xy(const double *__restrict__ bc, double *__restrict__ Fx,
                                                       double *__restrict__ Fxt, double *__restrict__ Fy,
@@ -286,12 +287,14 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Wa
         dt = A.ctl->dt;
     }
     __shared__ double smem[FV_SIZE + REG_SIZE];
-    if (blockIdx.z == 0) {
-        if ((int)blockIdx.x * TN > A.g.nx || (int)blockIdx.y * TT >= A.g.ny) return;
-        sweep7_tile<1, SCHEME, RIEMANN, BOTH>(bc, Fx, Fxt, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+    const int gx = (A.g.nx + 1 + TN - 1) / TN, tx = (A.g.ny + TT - 1) / TT;
+    const int gy = (A.g.ny + 1 + TN - 1) / TN;
+    const int id = blockIdx.x;
+    if (id < gx * tx) {
+        sweep7_tile<1, SCHEME, RIEMANN, BOTH>(bc, Fx, Fxt, A, mlp, dt, smem, id % gx, id / gx);
     } else {
-        if ((int)blockIdx.x * TN > A.g.ny || (int)blockIdx.y * TT >= A.g.nx) return;
-        sweep7_tile<2, SCHEME, RIEMANN, BOTH>(bc, Fy, Fyt, A, mlp, dt, smem, blockIdx.x, blockIdx.y);
+        const int iy = id - gx * tx;
+        sweep7_tile<2, SCHEME, RIEMANN, BOTH>(bc, Fy, Fyt, A, mlp, dt, smem, iy % gy, iy / gy);
     }
 }
 
@@ -531,7 +534,7 @@ static void launch7_t(const double *bc, double *Fx, double *Fxt, double *Fy, dou
     else if constexpr (SCHEME == SCHEME_SR32) mlp.p = (const float *)mlp_dev;
     const int gx = (A.g.nx + 1 + TN - 1) / TN, gy = (A.g.ny + 1 + TN - 1) / TN;
     const int tx = (A.g.ny + TT - 1) / TT, ty = (A.g.nx + TT - 1) / TT;
-    dim3 grid(gx > gy ? gx : gy, tx > ty ? tx : ty, 2);
+    dim3 grid(gx * tx + gy * ty);
     if (Fxt)
         hipLaunchKernelGGL((k7_sweep_xy<SCHEME, RIEMANN, true>), grid, dim3(SBLOCK), 0, st, bc, Fx, Fxt, Fy, Fyt, A,
                            mlp);
@@ -704,21 +707,25 @@ int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double 
     ws7_layout(cfg->nx, cfg->ny, &w, workspace);
     hipStream_t st = (hipStream_t)stream;
     Timer7 *tm = new Timer7();
-    tm->cap = 8 * n_steps;
+    tm->cap = 4 * n_steps;
     for (int i = 0; i < 2 * tm->cap; ++i)
         if (hipEventCreate(&tm->ev[i]) != hipSuccess) { delete tm; return WSR_E_HIP; }
+    int32_t step0 = 0, step1 = 0;
+    if (!loop_step_count(w.ctl, st, &step0)) rc = WSR_E_HIP;
     if (n_steps > 0) enqueue_params7(cfg, w, st);
     for (int n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(k7_dt, dim3(1), dim3(1), 0, st, w.ctl, cfg->loop_mode);
         enqueue_step7(cfg, w, u, 0.0, 0.0, w.ctl, dts, st, tm);
     }
-    rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
-    if (hipStreamSynchronize(st) != hipSuccess) rc = WSR_E_HIP;
+    if (rc == WSR_OK) rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+    if (!loop_step_count(w.ctl, st, &step1)) rc = WSR_E_HIP;
+    // only the launches of the steps that ran: four sweep launches per step, in order
+    int ran = step1 - step0;
+    ran = ran < 0 ? 0 : (ran > n_steps ? n_steps : ran);
     for (int k = 0; k < 2; ++k) { sweep_ms[k] = 0.0; launches[k] = 0; }
-    for (int i = 0; i < tm->used; ++i) {
+    for (int i = 0; i < tm->used && i < 4 * ran; ++i) {
         float ms = 0.0f;
-        // launches after the loop stopped return at once; count only the ones that did work
-        if (hipEventElapsedTime(&ms, tm->ev[2 * i], tm->ev[2 * i + 1]) == hipSuccess && ms > 0.01f) {
+        if (hipEventElapsedTime(&ms, tm->ev[2 * i], tm->ev[2 * i + 1]) == hipSuccess) {
             sweep_ms[tm->kind[i]] += ms;
             launches[tm->kind[i]] += 1;
         }

@@ -113,6 +113,17 @@ __device__ __forceinline__ void reduce_speed(double sp, bool active, Ctl *ctl) {
 
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// Steps the device time loop has completed (synchronises the stream first); the
+// timed loop entry points use it to count only the launches of steps that ran --
+// the launches enqueued after the loop stopped return at once.
+inline bool loop_step_count(const Ctl *ctl, hipStream_t st, int32_t *step) {
+    Ctl h;
+    if (hipStreamSynchronize(st) != hipSuccess) return false;
+    if (hipMemcpy(&h, ctl, sizeof(Ctl), hipMemcpyDeviceToHost) != hipSuccess) return false;
+    *step = h.step;
+    return true;
+}
+
 inline static float host_round_bf16(float x) {
     uint32_t u;
     memcpy(&u, &x, 4);
