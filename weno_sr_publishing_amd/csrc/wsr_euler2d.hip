@@ -1,6 +1,7 @@
 // wsr_euler2d.hip -- MI355X (gfx950) kernels and C ABI of the 2-D WENO5 step.
 //
-// One SSPRK3 stage is three launches:
+// One SSPRK3 stage is three launches (two in the device loop with a transmissive
+// or periodic boundary, where k_update also writes the next stage's ghost cells):
 //   k_boundary : ghost fill of the stage state into the workspace copy `bc`
 //                (transmissive / periodic / double-Mach).
 //   k_sweep_xy : the x and the y sweep in one launch.  The x tiles:
@@ -306,10 +307,52 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWav
 }
 
 // ------------------------------------------------------------------ update
+// Ghost images of interior index k (of n >= 3) under the transmissive (clamp) or
+// periodic (wrap) rule of k_boundary: the padded index of the cell itself, then the
+// ghost indices whose copy reads it -- at most 4 entries (clamp at an edge).
+constexpr int MAX_IMAGES = 4;
+__device__ __forceinline__ int ghost_images(int k, int n, int boundary, int out[MAX_IMAGES]) {
+    int m = 0;
+    out[m++] = k + 3;
+    if (boundary == WSR_BC_PERIODIC) {
+        if (k >= n - 3) out[m++] = k + 3 - n;
+        if (k < 3) out[m++] = k + 3 + n;
+    } else {
+        if (k == 0) { out[m++] = 0; out[m++] = 1; out[m++] = 2; }
+        if (k == n - 1) { out[m++] = n + 3; out[m++] = n + 4; out[m++] = n + 5; }
+    }
+    return m;
+}
+
+// Ghost fill of a stage array in place (transmissive / periodic), before the first
+// stage of a device-loop chunk; later stages get their ghosts from k_update.
+__global__ __launch_bounds__(BLOCK) void k_ghosts(double *u, Grid g, int boundary, const Ctl *ctl) {
+    if (ctl && ctl->done) return;
+    int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= g.nxt * g.nyt) return;
+    int j = idx / g.nxt, i = idx - j * g.nxt;
+    if (i >= 3 && i < g.nx + 3 && j >= 3 && j < g.ny + 3) return;
+    int si = i, sj = j;
+    if (boundary == WSR_BC_PERIODIC) {
+        if (i < 3) si = i + g.nx; else if (i >= g.nx + 3) si = i - g.nx;
+        if (j < 3) sj = j + g.ny; else if (j >= g.ny + 3) sj = j - g.ny;
+    } else {
+        if (i < 3) si = 3; else if (i >= g.nx + 3) si = g.nx + 2;
+        if (j < 3) sj = 3; else if (j >= g.ny + 3) sj = g.ny + 2;
+    }
+    const double2 *s2 = reinterpret_cast<const double2 *>(u + ((size_t)sj * g.nxt + si) * 4);
+    double2 a = s2[0], b = s2[1];
+    double2 *d2 = reinterpret_cast<double2 *>(u + (size_t)idx * 4);
+    d2[0] = a;
+    d2[1] = b;
+}
+
+// ghost_bc >= 0 (transmissive / periodic): u_out also receives its ghost cells, so the
+// next stage reads it without a boundary pass
 __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ us, const double *u0,
                                                   double *u_out, const double *__restrict__ Fx,
                                                   const double *__restrict__ Fy, Grid g, int rk,
-                                                  Ctl *ctl, int fold_speed, double *dts) {
+                                                  Ctl *ctl, int fold_speed, double *dts, int ghost_bc) {
     if (ctl && ctl->done) return;
     int idx = blockIdx.x * BLOCK + threadIdx.x;
     const bool active = idx < g.nx * g.ny;
@@ -330,6 +373,16 @@ __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ us,
             q[comp] = o;
         }
         for (int comp = 0; comp < 4; ++comp) u_out[c + comp] = q[comp];
+        if (ghost_bc >= 0 && (j < 3 || j >= g.ny - 3 || i < 3 || i >= g.nx - 3)) {
+            int rows[MAX_IMAGES], cols[MAX_IMAGES];
+            const int nr = ghost_images(j, g.ny, ghost_bc, rows), nc = ghost_images(i, g.nx, ghost_bc, cols);
+            for (int a = 0; a < nr; ++a)
+                for (int b = 0; b < nc; ++b) {
+                    if (a == 0 && b == 0) continue;
+                    double *o = u_out + ((size_t)rows[a] * g.nxt + cols[b]) * 4;
+                    o[0] = q[0]; o[1] = q[1]; o[2] = q[2]; o[3] = q[3];
+                }
+        }
     }
     if (fold_speed) {
         reduce_speed<BLOCK>(active ? cell_speed(q, g.gamma) : 0.0, active, ctl);
@@ -508,21 +561,43 @@ static void launch_sweep_xy(int scheme, int riemann, const double *bc, double *F
 }
 
 // one stage; ctl != nullptr selects the device-resident dt / time
+// In the device loop with a transmissive or periodic boundary the stage arrays carry
+// their own ghost cells (k_update writes them), so a stage reads u_stage directly.
+static bool loop_fills_ghosts(const wsr_euler2d_config *cfg) {
+    return (cfg->boundary == WSR_BC_TRANSMISSIVE || cfg->boundary == WSR_BC_PERIODIC) && cfg->nx >= 3 && cfg->ny >= 3;
+}
+
+// one stage; ctl != nullptr selects the device-resident dt / time; with own_ghosts
+// u_stage already holds its ghost cells and u_out receives them
 static void enqueue_stage(const wsr_euler2d_config *cfg, const Workspace &w, const double *u_stage,
                           const double *u0, double *u_out, double dt, int rk, double boundary_time,
-                          Ctl *ctl, int fold_speed, double *dts, hipStream_t st, SweepTimer *timer = nullptr) {
+                          Ctl *ctl, int fold_speed, double *dts, hipStream_t st, SweepTimer *timer = nullptr,
+                          bool own_ghosts = false) {
     Grid g = make_grid(cfg);
     const int ncells_p = g.nxt * g.nyt;
     double frac = rk == 1 ? 0.0 : (rk == 2 ? 1.0 : 0.5);
-    hipLaunchKernelGGL(k_boundary, dim3((ncells_p + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, u_stage, w.bc, g,
-                       cfg->boundary, boundary_time, (const Ctl *)ctl, frac);
+    const double *src = u_stage;
+    if (!own_ghosts) {
+        hipLaunchKernelGGL(k_boundary, dim3((ncells_p + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, u_stage, w.bc, g,
+                           cfg->boundary, boundary_time, (const Ctl *)ctl, frac);
+        src = w.bc;
+    }
     SweepArgs A;
     A.g = g; A.dt_host = dt; A.ctl = ctl; A.eno = cfg->eno_cutoff;
     A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
-    launch_sweep_xy(cfg->scheme, cfg->riemann, w.bc, w.Fx, w.Fy, A, w.mlp, st, timer);
+    launch_sweep_xy(cfg->scheme, cfg->riemann, src, w.Fx, w.Fy, A, w.mlp, st, timer);
     const int ncells = g.nx * g.ny;
-    hipLaunchKernelGGL(k_update, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const double *)w.bc, u0,
-                       u_out, (const double *)w.Fx, (const double *)w.Fy, g, rk, ctl, fold_speed, dts);
+    hipLaunchKernelGGL(k_update, dim3((ncells + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, src, u0, u_out,
+                       (const double *)w.Fx, (const double *)w.Fy, g, rk, ctl, fold_speed, dts,
+                       own_ghosts ? cfg->boundary : -1);
+}
+
+// ghost fill of u before the first stage of a device-loop chunk
+static void enqueue_loop_ghosts(const wsr_euler2d_config *cfg, const Workspace &w, double *u, hipStream_t st) {
+    Grid g = make_grid(cfg);
+    const int ncells_p = g.nxt * g.nyt;
+    hipLaunchKernelGGL(k_ghosts, dim3((ncells_p + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, u, g, cfg->boundary,
+                       (const Ctl *)w.ctl);
 }
 
 }  // namespace wsr
@@ -598,12 +673,14 @@ int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1,
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
     hipStream_t st = (hipStream_t)stream;
+    const bool own = loop_fills_ghosts(cfg);
     if (n_steps > 0) enqueue_params(cfg, w, st);
+    if (n_steps > 0 && own) enqueue_loop_ghosts(cfg, w, u, st);
     for (int n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
-        enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st);
-        enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st);
-        enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st);
+        enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st, nullptr, own);
+        enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st, nullptr, own);
+        enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st, nullptr, own);
     }
     return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
 }
@@ -625,12 +702,14 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
         if (hipEventCreate(&timer.ev[i]) != hipSuccess) return WSR_E_HIP;
     int32_t step0 = 0, step1 = 0;
     if (!loop_step_count(w.ctl, st, &step0)) rc = WSR_E_HIP;
+    const bool own = loop_fills_ghosts(cfg);
     if (n_steps > 0) enqueue_params(cfg, w, st);
+    if (n_steps > 0 && own) enqueue_loop_ghosts(cfg, w, u, st);
     for (int n = 0; n < n_steps; ++n) {
         hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
-        enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st, &timer);
-        enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st, &timer);
-        enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st, &timer);
+        enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st, &timer, own);
+        enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st, &timer, own);
+        enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st, &timer, own);
     }
     if (rc == WSR_OK) rc = hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
     if (!loop_step_count(w.ctl, st, &step1)) rc = WSR_E_HIP;
