@@ -14,6 +14,7 @@ Sources (relative to the reference checkout):
   experiments/02_sod_shock_tube/tables/sod_metrics.csv          (Sod N=100 density errors, 7 methods)
   experiments/03_lax_shock_tube/tables/lax_metrics.csv          (Lax N=200 primitive errors, 7 methods)
   experiments/06-09_riemann_c3..c6/tables/linecuts.csv          (C.3-C.6 density line cuts, 6 methods)
+  experiments/10_double_mach_reflection/tables/linecuts.csv     (double Mach line cuts vs WENO7-JS, 7 methods)
 """
 from __future__ import annotations
 
@@ -113,6 +114,8 @@ def linecuts() -> None:
                          ("c6", "09_riemann_c6")):
         rel = f"experiments/{folder}/tables/linecuts.csv"
         out[case] = {"source": rel, "rows": list(csv.DictReader((REF / rel).open()))}
+    rel = "experiments/10_double_mach_reflection/tables/linecuts.csv"
+    out["double_mach"] = {"source": rel, "rows": list(csv.DictReader((REF / rel).open()))}
     (OUT / "riemann_linecuts.json").write_text(json.dumps(out, indent=2) + "\n")
 
 
