@@ -61,6 +61,71 @@ def test_3d_fluxes_are_consistent(oracle, riemann):
             assert np.allclose(oracle.flux3d(q, q, d, 0.1, riemann), f, rtol=1e-13, atol=1e-13)
 
 
+def _prim5(rho, u, v, w, p):
+    return np.array([rho, rho * u, rho * v, rho * w, p / (GAMMA - 1) + 0.5 * rho * (u * u + v * v + w * w)])
+
+
+def _hllc_pairs(rng, count):
+    """Left / right primitive pairs spread over the HLLC cases: supersonic either way,
+    star state on either side of the contact (normal speeds up to Mach ~3)."""
+    for _ in range(count):
+        yield (rng.uniform(0.2, 3.0), rng.normal(0, 2.0), rng.normal(), rng.normal(), rng.uniform(0.1, 3.0)), \
+              (rng.uniform(0.2, 3.0), rng.normal(0, 2.0), rng.normal(), rng.normal(), rng.uniform(0.1, 3.0))
+
+
+def test_hllc5_rotational_invariance(oracle):
+    """The y and z fluxes are the x flux of the states with the normal velocity
+    moved into the x slot (and back): a wrong velocity index in the wave speeds,
+    the contact speed or the star momentum breaks it."""
+    rng = np.random.default_rng(11)
+    for (l, r) in _hllc_pairs(rng, 200):
+        for d, perm in ((2, [0, 2, 1, 3, 4]), (3, [0, 3, 2, 1, 4])):
+            ql, qr = _prim5(*l), _prim5(*r)
+            fd = oracle.flux3d(ql, qr, d, 0.1, "hllc")
+            fx = oracle.flux3d(ql[perm], qr[perm], 1, 0.1, "hllc")
+            assert np.allclose(fd, fx[perm], rtol=1e-14, atol=1e-14), (d, l, r)
+
+
+def test_hllc5_carries_tangential_velocity_with_the_mass_flux(oracle):
+    """In every HLLC region the tangential momentum fluxes are the mass flux times
+    the tangential velocity of the upwind side of the contact (the star states keep
+    the side's tangential velocities)."""
+    rng = np.random.default_rng(12)
+    regions = set()
+    for (l, r) in _hllc_pairs(rng, 300):
+        ql, qr = _prim5(*l), _prim5(*r)
+        f = oracle.flux3d(ql, qr, 1, 0.1, "hllc")
+        a_l, a_r = np.sqrt(GAMMA * l[4] / l[0]), np.sqrt(GAMMA * r[4] / r[0])
+        s_l, s_r = min(l[1] - a_l, r[1] - a_r), max(l[1] + a_l, r[1] + a_r)
+        if s_l >= 0.0 or s_r <= 0.0:
+            side = l if s_l >= 0.0 else r
+            regions.add("supersonic")
+        else:
+            num = r[4] - l[4] + l[0] * l[1] * (s_l - l[1]) - r[0] * r[1] * (s_r - r[1])
+            s_star = num / (l[0] * (s_l - l[1]) - r[0] * (s_r - r[1]))
+            side = l if s_star >= 0.0 else r
+            regions.add("star-left" if s_star >= 0.0 else "star-right")
+        for c in (2, 3):
+            assert f[c] == pytest.approx(f[0] * side[c], rel=1e-12, abs=1e-13), (l, r, c)
+    assert regions == {"supersonic", "star-left", "star-right"}
+
+
+def test_hllc5_resolves_a_stationary_contact(oracle):
+    """Equal pressure, zero normal velocity, any density and tangential velocities:
+    the flux is the pressure alone (HLLC keeps contacts exact)."""
+    rng = np.random.default_rng(13)
+    for _ in range(50):
+        p = rng.uniform(0.1, 3.0)
+        ql = _prim5(rng.uniform(0.2, 3.0), 0.0, rng.normal(), rng.normal(), p)
+        qr = _prim5(rng.uniform(0.2, 3.0), 0.0, rng.normal(), rng.normal(), p)
+        for d in (1, 2, 3):
+            perm = {1: [0, 1, 2, 3, 4], 2: [0, 2, 1, 3, 4], 3: [0, 3, 2, 1, 4]}[d]
+            f = oracle.flux3d(ql[perm], qr[perm], d, 0.1, "hllc")
+            want = np.zeros(5)
+            want[d] = p
+            assert np.allclose(f, want, rtol=0, atol=1e-14), (d, f)
+
+
 @pytest.mark.parametrize("scheme", ["js", "sr64"])
 def test_3d_uniform_flow(oracle, mlp64, scheme):
     n = 6
@@ -70,8 +135,8 @@ def test_3d_uniform_flow(oracle, mlp64, scheme):
     assert np.max(np.abs(out - q)) <= 1e-14
 
 
-@pytest.mark.parametrize("scheme", ["js", "sr64"])
-def test_3d_reduces_to_2d_for_z_invariant_states(oracle, mlp64, scheme):
+@pytest.mark.parametrize("scheme", ["js", "sr64", "sr32"])
+def test_3d_reduces_to_2d_for_z_invariant_states(oracle, mlp64, mlp32, scheme):
     nx, ny, nz = 20, 16, 5
     q2 = state3(nx, ny, 1)[0]                       # (ny+6, nx+6, 5) slice, w = 0 below
     q2[..., 3] = 0.0
@@ -79,7 +144,7 @@ def test_3d_reduces_to_2d_for_z_invariant_states(oracle, mlp64, scheme):
         0.5 * (q2[..., 1] ** 2 + q2[..., 2] ** 2) / q2[..., 0]
     q3 = np.ascontiguousarray(np.broadcast_to(q2, (nz + 6,) + q2.shape))
     dx, dy, dz, dt = 1.0 / nx, 1.0 / ny, 1.0 / nz, 2e-3
-    mlp = mlp64 if scheme == "sr64" else None
+    mlp = {"sr64": mlp64, "sr32": mlp32}.get(scheme)
     out3 = oracle.step3d(q3, nx, ny, nz, dx, dy, dz, dt, scheme=scheme, riemann="hllc", mlp=mlp)
     u2 = np.ascontiguousarray(q2[..., [0, 1, 2, 4]])
     out2 = oracle.step2d(u2, nx, ny, dx, dy, GAMMA, dt, scheme=scheme, riemann="hllc", boundary="transmissive",
