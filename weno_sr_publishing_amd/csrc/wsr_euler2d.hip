@@ -360,19 +360,34 @@ __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ us,
     if (active) {
         int j = idx / g.nx, i = idx - j * g.nx;
         size_t c = ((size_t)(j + 3) * g.nxt + (i + 3)) * 4;
-        const double *fxl = Fx + ((size_t)j * (g.nx + 1) + i) * 4;
-        const double *fyb = Fy + ((size_t)j * g.nx + i) * 4;
-        const double *fyt = Fy + ((size_t)(j + 1) * g.nx + i) * 4;
+        // 16-byte loads and stores: a cell's four components (and a face's fluxes)
+        // are 32-byte aligned records
+        double s[4], xl[4], xr[4], yb[4], yt[4], o0[4] = {0.0, 0.0, 0.0, 0.0};
+        auto ld4 = [](const double *p, double v[4]) {
+            const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+            v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+        };
+        ld4(us + c, s);
+        ld4(Fx + ((size_t)j * (g.nx + 1) + i) * 4, xl);
+        ld4(Fx + ((size_t)j * (g.nx + 1) + i + 1) * 4, xr);
+        ld4(Fy + ((size_t)j * g.nx + i) * 4, yb);
+        ld4(Fy + ((size_t)(j + 1) * g.nx + i) * 4, yt);
+        if (rk != 1) ld4(u0 + c, o0);
+        // the multiply-adds are spelled out so that the rounding does not depend on
+        // which product the compiler fuses: rhs = (u - h (F_r - F_l)) - h (G_t - G_b),
+        // u_out = b rhs + (a u0) with one rounding of the b rhs product
 #pragma unroll
         for (int comp = 0; comp < 4; ++comp) {
-            double rhs = us[c + comp] - 0.5 * (fxl[4 + comp] - fxl[comp]) - 0.5 * (fyt[comp] - fyb[comp]);
+            double rhs = fma(-0.5, yt[comp] - yb[comp], fma(-0.5, xr[comp] - xl[comp], s[comp]));
             double o;
             if (rk == 1) o = rhs;
-            else if (rk == 2) o = 0.75 * u0[c + comp] + 0.25 * rhs;
-            else o = (1.0 / 3.0) * u0[c + comp] + (2.0 / 3.0) * rhs;
+            else if (rk == 2) o = fma(0.25, rhs, 0.75 * o0[comp]);
+            else o = fma(2.0 / 3.0, rhs, (1.0 / 3.0) * o0[comp]);
             q[comp] = o;
         }
-        for (int comp = 0; comp < 4; ++comp) u_out[c + comp] = q[comp];
+        double2 *d2 = reinterpret_cast<double2 *>(u_out + c);
+        d2[0] = make_double2(q[0], q[1]);
+        d2[1] = make_double2(q[2], q[3]);
         if (ghost_bc >= 0 && (j < 3 || j >= g.ny - 3 || i < 3 || i >= g.nx - 3)) {
             int rows[MAX_IMAGES], cols[MAX_IMAGES];
             const int nr = ghost_images(j, g.ny, ghost_bc, rows), nc = ghost_images(i, g.nx, ghost_bc, cols);
