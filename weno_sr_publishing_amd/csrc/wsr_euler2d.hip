@@ -400,7 +400,7 @@ __global__ __launch_bounds__(BLOCK) void k_update(const double *__restrict__ us,
         }
     }
     if (fold_speed) {
-        reduce_speed<BLOCK>(active ? cell_speed(q, g.gamma) : 0.0, active, ctl);
+        reduce_speed<BLOCK, true>(active ? cell_speed(q, g.gamma) : 0.0, active, ctl);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             // run_to_time bookkeeping: t += dt; step += 1; dts.append(dt)
             double dt = ctl->dt;
@@ -427,6 +427,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_speed(const double *__restrict__ 
 
 __global__ void k_reset_speed(Ctl *ctl) {
     ctl->speed_bits = 0ull;
+    for (int i = 0; i < SPEED_SLOTS; ++i) ctl->speed_slot[i * SPEED_SLOT_STRIDE] = 0ull;
     ctl->nan_flag = 0;
 }
 
@@ -434,18 +435,33 @@ __global__ void k_loop_init(Ctl *ctl, double t_end, double cfl_h, int max_steps)
     ctl->t = 0.0; ctl->dt = 0.0; ctl->t_end = t_end; ctl->cfl_h = cfl_h; ctl->dt_last = 0.0;
     ctl->speed_bits = 0ull; ctl->step = 0; ctl->done = 0; ctl->nan_flag = 0; ctl->max_steps = max_steps;
     for (int i = 0; i < NCOUNT; ++i) ctl->counters[i] = 0ull;
+    for (int i = 0; i < SPEED_SLOTS; ++i) ctl->speed_slot[i * SPEED_SLOT_STRIDE] = 0ull;
 }
 
-// dt of the next step (warp_weno5_helpers.py:189-207 and the loop head of run_to_time)
-__global__ void k_dt(Ctl *ctl) {
+// dt of the next step (warp_weno5_helpers.py:189-207 and the loop head of run_to_time).
+// One wave: the max speed is speed_bits (k_max_speed) folded with the slots of the
+// fused reduction in k_update (the max of non-negative IEEE bit patterns); lane 0
+// applies the rules, every lane clears its slot.
+constexpr int DT_THREADS = 64;
+static_assert(SPEED_SLOTS <= DT_THREADS, "one lane per speed slot");
+__global__ __launch_bounds__(DT_THREADS) void k_dt(Ctl *ctl) {
     if (ctl->done) return;
+    const int lane = threadIdx.x;
+    unsigned long long bits = lane < SPEED_SLOTS ? ctl->speed_slot[lane * SPEED_SLOT_STRIDE] : 0ull;
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_down(bits, off);
+        bits = o > bits ? o : bits;
+    }
+    if (lane < SPEED_SLOTS) ctl->speed_slot[lane * SPEED_SLOT_STRIDE] = 0ull;
+    if (lane != 0) return;
+    bits = ctl->speed_bits > bits ? ctl->speed_bits : bits;
     double t = ctl->t;
     if (!(t < ctl->t_end - 1.0e-14) || ctl->step >= ctl->max_steps) { ctl->done = 1; return; }
     double dt;
     if (ctl->nan_flag) {
         dt = __longlong_as_double(0x7ff8000000000000ll);
     } else {
-        double max_speed = __longlong_as_double((long long)ctl->speed_bits);
+        double max_speed = __longlong_as_double((long long)bits);
         dt = max_speed < 1.0e-15 ? 1.0e10 : ctl->cfl_h / max_speed;
     }
     double rem = ctl->t_end - t;
@@ -692,7 +708,7 @@ int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1,
     if (n_steps > 0) enqueue_params(cfg, w, st);
     if (n_steps > 0 && own) enqueue_loop_ghosts(cfg, w, u, st);
     for (int n = 0; n < n_steps; ++n) {
-        hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
+        hipLaunchKernelGGL(k_dt, dim3(1), dim3(DT_THREADS), 0, st, w.ctl);
         enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st, nullptr, own);
         enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st, nullptr, own);
         enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st, nullptr, own);
@@ -721,7 +737,7 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
     if (n_steps > 0) enqueue_params(cfg, w, st);
     if (n_steps > 0 && own) enqueue_loop_ghosts(cfg, w, u, st);
     for (int n = 0; n < n_steps; ++n) {
-        hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, st, w.ctl);
+        hipLaunchKernelGGL(k_dt, dim3(1), dim3(DT_THREADS), 0, st, w.ctl);
         enqueue_stage(cfg, w, u, u, u1, 0.0, 1, 0.0, w.ctl, 0, dts, st, &timer, own);
         enqueue_stage(cfg, w, u1, u, u2, 0.0, 2, 0.0, w.ctl, 0, dts, st, &timer, own);
         enqueue_stage(cfg, w, u2, u, u, 0.0, 3, 0.0, w.ctl, 1, dts, st, &timer, own);

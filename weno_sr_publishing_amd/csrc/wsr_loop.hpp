@@ -12,6 +12,7 @@
 namespace wsr {
 
 constexpr int NCOUNT = 6;
+constexpr int SPEED_SLOTS = 16, SPEED_SLOT_STRIDE = 16;
 
 struct Ctl {
     double t, dt, t_end, cfl_h, dt_last;
@@ -22,6 +23,10 @@ struct Ctl {
     // reconstructions, [3] of which evaluated the closure, [4] / [5] lanes of
     // the waves that ran the closure code (normal / Gauss; 64 per wave pass)
     unsigned long long counters[NCOUNT];
+    // per-block maxima of the fused speed reduction, spread over SPEED_SLOTS
+    // addresses on separate 128-byte lines (one address would serialise every
+    // block's atomic in one L2 channel); the dt kernel folds them into speed_bits
+    unsigned long long speed_slot[SPEED_SLOTS * SPEED_SLOT_STRIDE];
 };
 
 struct Grid {
@@ -86,7 +91,7 @@ __device__ __forceinline__ double cell_speed(const double q[4], double gamma) {
 }
 
 // block max of non-negative speeds (NaN reported separately) -> atomics in ctl
-template <int NTHREADS>
+template <int NTHREADS, bool SLOTS = false>
 __device__ __forceinline__ void reduce_speed(double sp, bool active, Ctl *ctl) {
     __shared__ unsigned long long red[NTHREADS / 64];
     __shared__ int nan_seen;
@@ -106,7 +111,8 @@ __device__ __forceinline__ void reduce_speed(double sp, bool active, Ctl *ctl) {
     if (threadIdx.x == 0) {
         unsigned long long m = red[0];
         for (int w = 1; w < NTHREADS / 64; ++w) m = red[w] > m ? red[w] : m;
-        atomicMax(&ctl->speed_bits, m);
+        if constexpr (SLOTS) atomicMax(&ctl->speed_slot[(blockIdx.x % SPEED_SLOTS) * SPEED_SLOT_STRIDE], m);
+        else atomicMax(&ctl->speed_bits, m);
         if (nan_seen) atomicOr(&ctl->nan_flag, 1);
     }
 }
