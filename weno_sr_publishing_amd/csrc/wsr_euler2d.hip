@@ -119,7 +119,7 @@ struct SweepArgs {
     int eno;
 };
 
-// wave-level count of lanes with `flag` set, added to *dst by one lane
+// face item -> (face within the tile, transverse line)
 template <int DIR>
 __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
     if (DIR == 1) { fl = it % TN; tau = it / TN; }      // faces fastest (x: contiguous cells)
@@ -235,22 +235,24 @@ __device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double
         for (int base = 0; base < CH; base += SBLOCK / 8) {
             const int ci = base + slot;
             const int it = c0 + ci;
-            const bool valid = it < N2;
+            // items past the tile's N2 faces (whole waves of the last pass at 4 x 28)
+            // have no face: no reconstruction, no flux
+            if (it >= N2) continue;
             int fl, tl;
-            item_map<DIR>(valid ? it : N2 - 1, TT, fl, tl);
+            item_map<DIR>(it, TT, fl, tl);
             double q[5];
 #pragma unroll
             for (int m = 0; m < 5; ++m) q[m] = FV[((tl + m) * TN + fl) * 8 + gi];
             double g1, g2;
             bool evaluated;
             recon_gauss<SCHEME>(q, mlp, A.eno, h_t, g1, g2, evaluated);
-            if (A.counters) cnt.add(2, 5, valid && (f0 + fl) < nfaces && (t0 + tl) < ntrans, evaluated);
+            if (A.counters) cnt.add(2, 5, (f0 + fl) < nfaces && (t0 + tl) < ntrans, evaluated);
             GV[ci * 16 + gi] = g1;
             GV[ci * 16 + 8 + gi] = g2;
         }
         __syncthreads();
         // 2b: one flux per thread: (face ci, Gauss point gp)
-        {
+        if (c0 + (int)(threadIdx.x >> 1) < N2) {
             const int ci = threadIdx.x >> 1, gp = threadIdx.x & 1;
             const double *v = GV + ci * 16 + gp * 8;
             double ul[4] = {v[0], v[1], v[2], v[3]};
