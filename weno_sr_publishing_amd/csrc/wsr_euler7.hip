@@ -338,34 +338,81 @@ __global__ __launch_bounds__(BLOCK) void k7_stage(StageArrays S, Grid g, double 
         int j = idx / g.nx, i = idx - j * g.nx;
         size_t c = ((size_t)(j + GC) * g.nxt + (i + GC)) * 4;
         const double inv_dx = 1.0 / g.dx, inv_dy = 1.0 / g.dy;
+        // every input is loaded (as 16-byte halves of the 32-byte cell and face
+        // records) before the first store: the stage arrays are distinct, but the
+        // compiler cannot know it and would otherwise wait out each component's
+        // stores before the next component's loads.  The multiply-adds are spelled
+        // out so that the rounding does not depend on the compiler's choice of the
+        // product to fuse (a sum of two products fuses the second, a difference the
+        // first).
+        auto ld4 = [](const double *p, double v[4]) {
+            const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+            v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+        };
+        auto st4 = [](double *p, const double v[4]) {
+            reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+            reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+        };
+        // rhs_at for the four components of cell (j, i)
+        auto rhs4 = [&](const double *Fx, const double *Fy, double out[4]) {
+            double xl[4], xr[4], yb[4], yt[4];
+            ld4(Fx + ((size_t)j * (g.nx + 1) + i) * 4, xl);
+            ld4(Fx + ((size_t)j * (g.nx + 1) + i + 1) * 4, xr);
+            ld4(Fy + ((size_t)j * g.nx + i) * 4, yb);
+            ld4(Fy + ((size_t)(j + 1) * g.nx + i) * 4, yt);
 #pragma unroll
-        for (int comp = 0; comp < 4; ++comp) {
-            const size_t e = c + comp;
-            double r = rhs_at(S.Fx, S.Fy, g, j, i, comp, inv_dx, inv_dy);
-            if constexpr (STAGE == 0) {
-                double rt = rhs_at(S.Fxt, S.Fyt, g, j, i, comp, inv_dx, inv_dy);
-                double u = S.u[e];
-                S.u1[e] = u + 0.5 * dt * r;
-                S.acc[e] = 1.0 / 5.0 * u + 1.0 / 10.0 * dt * r;
-                S.rt0[e] = rt;
-            } else if constexpr (STAGE == 1) {
-                double rt1 = rhs_at(S.Fxt, S.Fyt, g, j, i, comp, inv_dx, inv_dy);
-                double u = S.u[e], u1 = S.u1[e], rt0 = S.rt0[e];
-                double u2 = 649.0 / 1600.0 * u - 10890423.0 / 25193600.0 * dt * rt0 + 951.0 / 1600.0 * u1 +
-                            5000.0 / 7873.0 * dt * r;
-                S.u2[e] = u2;
-                S.acc[e] = S.acc[e] + 6127.0 / 30000.0 * u1 + 1.0 / 6.0 * dt * r + 7873.0 / 30000.0 * u2;
-                S.u3[e] = 53989.0 / 2500000.0 * u - 102261.0 / 5000000.0 * dt * rt0 +
-                          4806213.0 / 20000000.0 * u1 - 5121.0 / 20000.0 * dt * rt1;
-            } else if constexpr (STAGE == 2) {
-                double u3 = S.u3[e] + 23619.0 / 32000.0 * S.u2[e] + 7873.0 / 10000.0 * dt * r;
-                S.u3[e] = u3;
-                S.acc[e] = S.acc[e] + 1.0 / 3.0 * u3;
-            } else {
-                double un = S.acc[e] + 1.0 / 6.0 * dt * r;
-                S.u[e] = un;
-                q[comp] = un;
+            for (int k = 0; k < 4; ++k) out[k] = fma(-0.5 * (xr[k] - xl[k]), inv_dx, -0.5 * (yt[k] - yb[k]) * inv_dy);
+        };
+        double r[4];
+        rhs4(S.Fx, S.Fy, r);
+        if constexpr (STAGE == 0) {
+            double rt[4], u[4], u1[4], acc[4];
+            rhs4(S.Fxt, S.Fyt, rt);
+            ld4(S.u + c, u);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                u1[k] = fma(0.5 * dt, r[k], u[k]);
+                acc[k] = fma(1.0 / 10.0 * dt, r[k], 1.0 / 5.0 * u[k]);
             }
+            st4(S.u1 + c, u1);
+            st4(S.acc + c, acc);
+            st4(S.rt0 + c, rt);
+        } else if constexpr (STAGE == 1) {
+            double rt1[4], u[4], u1[4], rt0[4], acc[4], u2[4], u3[4];
+            rhs4(S.Fxt, S.Fyt, rt1);
+            ld4(S.u + c, u);
+            ld4(S.u1 + c, u1);
+            ld4(S.rt0 + c, rt0);
+            ld4(S.acc + c, acc);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                u2[k] = fma(5000.0 / 7873.0 * dt, r[k],
+                            fma(951.0 / 1600.0, u1[k], fma(649.0 / 1600.0, u[k], -(10890423.0 / 25193600.0 * dt * rt0[k]))));
+                acc[k] = fma(7873.0 / 30000.0, u2[k], fma(1.0 / 6.0 * dt, r[k], fma(6127.0 / 30000.0, u1[k], acc[k])));
+                u3[k] = fma(-(5121.0 / 20000.0 * dt), rt1[k],
+                            fma(4806213.0 / 20000000.0, u1[k], fma(53989.0 / 2500000.0, u[k], -(102261.0 / 5000000.0 * dt * rt0[k]))));
+            }
+            st4(S.u2 + c, u2);
+            st4(S.acc + c, acc);
+            st4(S.u3 + c, u3);
+        } else if constexpr (STAGE == 2) {
+            double u3[4], u2[4], acc[4];
+            ld4(S.u3 + c, u3);
+            ld4(S.u2 + c, u2);
+            ld4(S.acc + c, acc);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                u3[k] = fma(7873.0 / 10000.0 * dt, r[k], fma(23619.0 / 32000.0, u2[k], u3[k]));
+                acc[k] = fma(1.0 / 3.0, u3[k], acc[k]);
+            }
+            st4(S.u3 + c, u3);
+            st4(S.acc + c, acc);
+        } else {
+            double acc[4];
+            ld4(S.acc + c, acc);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) q[k] = fma(1.0 / 6.0 * dt, r[k], acc[k]);
+            st4(S.u + c, q);
         }
     }
     if constexpr (STAGE == 3) {
