@@ -46,7 +46,8 @@ enum {
 /* error codes */
 enum {
     WSR_OK = 0,
-    WSR_E_ARG = -1,        /* bad size / enum / null pointer */
+    WSR_E_ARG = -1,        /* bad size / enum / null pointer; a 2-D or WENO7 state, buffer or workspace
+                              pointer that is not 16-byte aligned (cells and faces move as 16-byte halves) */
     WSR_E_MLP = -2,        /* SR scheme without MLP parameters */
     WSR_E_WORKSPACE = -3,  /* workspace too small */
     WSR_E_HIP = -4         /* a HIP runtime call failed */

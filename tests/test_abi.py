@@ -49,7 +49,11 @@ def test_workspace_size_and_argument_checks():
     cfg.scheme = 2  # SR32 without parameters
     assert L.wsr_euler2d_stage(ctypes.byref(cfg), 1, 1, 1, 0.1, 1, 0.0, 1, 1 << 20, None) == -2
     cfg.scheme = 0
-    assert L.wsr_euler2d_stage(ctypes.byref(cfg), 1, 1, 1, 0.1, 1, 0.0, 1, 16, None) == -3
+    a = 1 << 12     # stand-in addresses: never dereferenced, the checks return first
+    assert L.wsr_euler2d_stage(ctypes.byref(cfg), a, a, a, 0.1, 1, 0.0, a, 16, None) == -3
+    # the kernels move 32-byte records as 16-byte halves: misaligned pointers are refused
+    assert L.wsr_euler2d_stage(ctypes.byref(cfg), a + 8, a, a, 0.1, 1, 0.0, a, 16, None) == -1
+    assert L.wsr_euler2d_stage(ctypes.byref(cfg), a, a, a, 0.1, 1, 0.0, a + 8, 16, None) == -1
     cfg.boundary = 7
     assert L.wsr_euler2d_stage(ctypes.byref(cfg), 1, 1, 1, 0.1, 1, 0.0, 1, 1 << 20, None) == -1
 

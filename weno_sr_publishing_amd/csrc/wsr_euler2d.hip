@@ -650,6 +650,7 @@ int wsr_euler2d_stage(const wsr_euler2d_config *cfg, const double *u_stage, cons
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (!u_stage || !u0 || !u_out || !workspace || rk < 1 || rk > 3) return WSR_E_ARG;
+    if (!all_aligned16(u_stage, u0, u_out, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -663,6 +664,7 @@ int wsr_euler2d_max_speed(const wsr_euler2d_config *cfg, const double *u, void *
                           double *max_speed_out, void *stream) {
     if (!cfg || cfg->ghost != 3 || cfg->nx < 1 || cfg->ny < 1) return WSR_E_ARG;
     if (!u || !workspace || !max_speed_out) return WSR_E_ARG;
+    if (!all_aligned16(u, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -685,6 +687,7 @@ int wsr_euler2d_loop_init(const wsr_euler2d_config *cfg, const double *u, int32_
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (!u || !workspace || max_steps < 0) return WSR_E_ARG;
+    if (!all_aligned16(u, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -702,6 +705,7 @@ int wsr_euler2d_loop_steps(const wsr_euler2d_config *cfg, double *u, double *u1,
     int rc = check_cfg(cfg);
     if (rc) return rc;
     if (!u || !u1 || !u2 || !dts || !workspace || n_steps < 0) return WSR_E_ARG;
+    if (!all_aligned16(u, u1, u2, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -725,6 +729,7 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
     if (rc) return rc;
     if (!u || !u1 || !u2 || !dts || !workspace || n_steps < 0 || n_steps > 64 || !sweep_ms || !sweep_launches)
         return WSR_E_ARG;
+    if (!all_aligned16(u, u1, u2, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);

@@ -683,6 +683,7 @@ int wsr_euler7_rhs(const wsr_euler7_config *cfg, const double *u, double *rhs, d
     int rc = check7(cfg);
     if (rc) return rc;
     if (!u || !rhs || !workspace) return WSR_E_ARG;
+    if (!all_aligned16(u, rhs, rhs_reverse, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws7_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace7 w;
     ws7_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -701,6 +702,7 @@ int wsr_euler7_step(const wsr_euler7_config *cfg, double *u, double dt, double t
     int rc = check7(cfg);
     if (rc) return rc;
     if (!u || !workspace) return WSR_E_ARG;
+    if (!all_aligned16(u, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws7_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace7 w;
     ws7_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -715,6 +717,7 @@ int wsr_euler7_loop_init(const wsr_euler7_config *cfg, const double *u, int32_t 
     int rc = check7(cfg);
     if (rc) return rc;
     if (!u || !workspace || max_steps < 0) return WSR_E_ARG;
+    if (!all_aligned16(u, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws7_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace7 w;
     ws7_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -732,6 +735,7 @@ int wsr_euler7_loop_steps(const wsr_euler7_config *cfg, double *u, double *dts, 
     int rc = check7(cfg);
     if (rc) return rc;
     if (!u || !dts || !workspace || n_steps < 0) return WSR_E_ARG;
+    if (!all_aligned16(u, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws7_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace7 w;
     ws7_layout(cfg->nx, cfg->ny, &w, workspace);
@@ -751,6 +755,7 @@ int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double 
     if (rc) return rc;
     if (!u || !dts || !workspace || n_steps < 0 || n_steps > TIMER7_MAX_STEPS || !sweep_ms || !launches)
         return WSR_E_ARG;
+    if (!all_aligned16(u, workspace)) return WSR_E_ARG;
     if (workspace_bytes < ws7_layout(cfg->nx, cfg->ny, nullptr, nullptr)) return WSR_E_WORKSPACE;
     Workspace7 w;
     ws7_layout(cfg->nx, cfg->ny, &w, workspace);

@@ -119,6 +119,12 @@ __device__ __forceinline__ void reduce_speed(double sp, bool active, Ctl *ctl) {
 
 inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
+// State, buffer and workspace pointers handed to the C ABI must be 16-byte aligned:
+// the kernels move the 32-byte cell and face records as 16-byte halves.  Null
+// pointers pass (each entry point checks the ones it needs separately).
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
+template <class... P> inline bool all_aligned16(const P *...p) { return (aligned16(p) && ...); }
+
 // Steps the device time loop has completed (synchronises the stream first); the
 // timed loop entry points use it to count only the launches of steps that ran --
 // the launches enqueued after the loop stopped return at once.
