@@ -142,6 +142,7 @@ class Euler2DSolver:
         _lib.check(rc, "wsr_euler2d_loop_init")
 
     def loop_steps(self, u: torch.Tensor, n: int) -> None:
+        self._check_state(u)
         rc = self._lib.wsr_euler2d_loop_steps(ctypes.byref(self.cfg), u.data_ptr(), self._u1.data_ptr(),
                                               self._u2.data_ptr(), self._dts.data_ptr(), int(n),
                                               self.workspace.data_ptr(), self.ws_bytes, self._stream())
@@ -149,6 +150,7 @@ class Euler2DSolver:
 
     def loop_steps_timed(self, u: torch.Tensor, n: int) -> tuple[float, int]:
         """loop_steps with HIP events around every sweep launch; returns (ms, launches)."""
+        self._check_state(u)
         ms = ctypes.c_double(0.0)
         launches = ctypes.c_int32(0)
         rc = self._lib.wsr_euler2d_loop_steps_timed(

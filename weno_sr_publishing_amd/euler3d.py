@@ -221,6 +221,7 @@ class Euler3Solver:
         _lib.check(rc, "wsr_euler3_loop_init")
 
     def loop_steps(self, q: torch.Tensor, n: int) -> None:
+        self._check(q)
         rc = self._lib.wsr_euler3_loop_steps(ctypes.byref(self.cfg), q.data_ptr(), self._dts.data_ptr(), int(n),
                                              self.workspace.data_ptr(), self.ws_bytes, self._stream())
         _lib.check(rc, "wsr_euler3_loop_steps")
@@ -228,6 +229,7 @@ class Euler3Solver:
     def loop_steps_timed(self, q: torch.Tensor, n: int) -> tuple[list[float], list[int]]:
         """n (<= 32) steps with HIP events around the normal / first-pass / flux
         launches: ([ms] x 3, [launches] x 3) of the launches that did work."""
+        self._check(q)
         ms = (ctypes.c_double * 3)()
         cnt = (ctypes.c_int32 * 3)()
         rc = self._lib.wsr_euler3_loop_steps_timed(ctypes.byref(self.cfg), q.data_ptr(), self._dts.data_ptr(),

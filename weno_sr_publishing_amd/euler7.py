@@ -160,6 +160,7 @@ class Euler7Solver:
         _lib.check(rc, "wsr_euler7_loop_init")
 
     def loop_steps(self, u: torch.Tensor, n: int) -> None:
+        self._check(u)
         rc = self._lib.wsr_euler7_loop_steps(ctypes.byref(self.cfg), u.data_ptr(), self._dts.data_ptr(), int(n),
                                              self.workspace.data_ptr(), self.ws_bytes, self._stream())
         _lib.check(rc, "wsr_euler7_loop_steps")
@@ -167,6 +168,7 @@ class Euler7Solver:
     def loop_steps_timed(self, u: torch.Tensor, n: int) -> tuple[list[float], list[int]]:
         """n (<= 32) steps with HIP events around every sweep launch: ([ms] x 2,
         [launches] x 2) for the L-and-L~ sweeps and the L-only sweeps that did work."""
+        self._check(u)
         ms = (ctypes.c_double * 2)()
         cnt = (ctypes.c_int32 * 2)()
         rc = self._lib.wsr_euler7_loop_steps_timed(ctypes.byref(self.cfg), u.data_ptr(), self._dts.data_ptr(),
