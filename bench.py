@@ -722,6 +722,13 @@ def parse():
                          "trajectories x 256 cells, 64-cell rollout, hand-written adjoint); gste1d: the 1-D "
                          "GSTE advection run (N=200, t=10); each prints its own JSON line")
     ap.add_argument("--no-accuracy", action="store_true")
+    ap.add_argument("--no-full", action="store_true",
+                    help="with --steps N: skip the full-trajectory runs (full_trajectory = null)")
+    ap.add_argument("--save-start", metavar="NPZ",
+                    help="with --steps N: write the developed start states (t=0.125) of both closures and exit")
+    ap.add_argument("--start-from", metavar="NPZ",
+                    help="with --steps N: load the start states written by --save-start instead of advancing "
+                         "from the initial data (a profiled run then holds only the window's kernels)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target CPU-baseline sample length")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -792,12 +799,27 @@ def main() -> int:
     windowed = args.steps > 0
     t_mid = T_MID if windowed else 0.0
     starts = {}
+    loaded = None
+    if windowed and args.start_from:
+        with np.load(args.start_from, allow_pickle=False) as z:
+            loaded = {k: z[k] for k in z.files}
+        if float(loaded["t_mid"]) != t_mid:
+            raise SystemExit(f"{args.start_from}: start states are at t={float(loaded['t_mid'])}, not {t_mid}")
     for scheme in (args.closure, "js"):
         s = make(scheme)
-        u = s.new_state(u0_host)
-        if windowed:
-            s.run(u, t_mid)
+        if loaded is not None:
+            u = s.new_state(loaded[scheme])
+        else:
+            u = s.new_state(u0_host)
+            if windowed:
+                s.run(u, t_mid)
         starts[scheme] = u
+    if windowed and args.save_start:
+        np.savez(args.save_start, t_mid=np.float64(t_mid),
+                 **{k: v.cpu().numpy() for k, v in starts.items()})
+        print(json.dumps({"saved_start_states": args.save_start, "t_mid": t_mid, "closures": list(starts)}),
+              flush=True)
+        return 0
     max_steps = args.steps if windowed else 10_000_000
 
     def timed(scheme: str, instrument: bool):
@@ -845,7 +867,9 @@ def main() -> int:
 
     # the whole trajectory t = 0 .. T_END from the IC for both closures (extra keys;
     # with --steps 0 it is the timed region itself)
-    if windowed:
+    if windowed and args.no_full:
+        full = {}
+    elif windowed:
         full = {}
         for scheme in (args.closure, "js"):
             s = make(scheme)
@@ -854,7 +878,7 @@ def main() -> int:
             full[scheme] = (u, n, w)
     else:
         full = {args.closure: (u_sr, steps_sr, wall_sr), "js": (u_js, steps_js, wall_js)}
-    f_sr, f_js = full[args.closure], full["js"]
+    f_sr, f_js = full.get(args.closure), full.get("js")
     window = (f"t={t_from:.4f}..{t_to:.4f} ({steps_sr} steps from the developed flow)" if windowed
               else f"t=0..{T_END} (full trajectory, {steps_sr} steps)")
 
@@ -880,7 +904,7 @@ def main() -> int:
         "wall_s_event_instrumented": {"sr": wall_i, "js": wall_js_i},
         "js": {"value": js_value, "steps": steps_js, "wall_s": wall_js, "ms_per_step": 1e3 * wall_js / steps_js},
         "sr_vs_js_wall": wall_sr / wall_js,
-        "full_trajectory": {
+        "full_trajectory": None if f_sr is None else {
             "t": [0.0, T_END],
             "sr": {"value": NX * NY * f_sr[1] / f_sr[2], "steps": f_sr[1], "wall_s": f_sr[2]},
             "js": {"value": NX * NY * f_js[1] / f_js[2], "steps": f_js[1], "wall_s": f_js[2]},
@@ -888,7 +912,7 @@ def main() -> int:
         },
         "reference_rtx5080_wall_s": {"sr32": REFERENCE_SR32_WALL_S, "js": REFERENCE_JS_WALL_S},
         "speedup_vs_reference_wall": ({"sr": REFERENCE_SR32_WALL_S / f_sr[2], "js": REFERENCE_JS_WALL_S / f_js[2]}
-                                      if args.closure == "sr32" else None),
+                                      if args.closure == "sr32" and f_sr is not None else None),
         "roofline": {
             "bound": "valu",
             "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
