@@ -781,15 +781,17 @@ def main() -> int:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         s.loop_init(u, t_span, max_steps)
+        steps = 0
         while True:
+            chunk = min(64, max_steps - steps)   # no launches past the last step
             if instrument:
-                ms, n = s.loop_steps_timed(u, 64)
+                ms, n = s.loop_steps_timed(u, chunk)
                 sweep_ms += ms
                 sweep_n += n
             else:
-                s.loop_steps(u, 64)
+                s.loop_steps(u, chunk)
             steps, t, _, done = s.loop_status()
-            if done:
+            if done or steps >= max_steps:
                 break
         torch.cuda.synchronize()
         return steps, t, time.perf_counter() - t0, sweep_ms, sweep_n
