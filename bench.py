@@ -374,7 +374,13 @@ def bench_doublemach7(args) -> int:
         torch.cuda.synchronize()
         return s, u, n, t, time.perf_counter() - t0, ms, nl
 
-    _, u_sr, n_sr, t_sr, wall_sr, _, _ = timed(closure)
+    s_sr, u_sr, n_sr, t_sr, wall_sr, _, _ = timed(closure)
+    fallbacks = s_sr.fallbacks()
+    # reconstructed states per step that the fallback counter can count: four
+    # right-hand sides (L~ shares L's reconstructions); normal face states (face, side)
+    # on the Gauss stencils' lines and Gauss-point states (face, side, point)
+    states_rhs = (2 * ((NX7 + 1) * (NY7 + 6) + (NY7 + 1) * (NX7 + 6))
+                  + 4 * ((NX7 + 1) * NY7 + (NY7 + 1) * NX7))
     _, _, n_js, t_js, wall_js, _, _ = timed("js")
     stats = interior_stats(u_sr.cpu().numpy(), params)
     _, _, _, _, wall_i, kms, kn = timed(closure, instrument=True)
@@ -413,6 +419,10 @@ def bench_doublemach7(args) -> int:
                    "boundary": "double-mach", "t_end": t_sr},
         "wall_s": wall_sr,
         "health": stats,
+        "z_fallback": {"states": fallbacks, "fraction_of_states": fallbacks / max(1, 4 * states_rhs * n_sr),
+                       "criterion": "the closure's dominant candidate has a WENO7-Z indicator above 50x the "
+                                    "smallest + 240 h^3, or a ratio is not finite / outside [0, 1]; the whole "
+                                    "state (4 components) is then reconstructed with WENO7-Z"},
         "wall_s_event_instrumented": {"sr": wall_i, "js": wall_js_i},
         "js": {"value": ncell * n_js / wall_js, "steps": n_js, "wall_s": wall_js,
                "ms_per_step": 1e3 * wall_js / n_js},
@@ -452,7 +462,7 @@ def bench_doublemach7(args) -> int:
         while True:
             um, _, k, _ = oracle.run7_2d(um, NX7, NY7, params.dx, params.dy, params.gamma, params.cfl, T7,
                                          scheme=closure, riemann="hllc", boundary="double-mach",
-                                         mlp=mlps[closure], mode="double-mach", max_steps=1)
+                                         mlp=mlps[closure], mode="double-mach", max_steps=1, z_fallback=True)
             done_steps += k
             el = time.perf_counter() - t0
             if k == 0 or el >= args.cpu_seconds or done_steps >= 50:
@@ -460,7 +470,7 @@ def bench_doublemach7(args) -> int:
         cpu_rate = ncell * done_steps / el
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "cell-updates/s", "cores": threads, "kind": "port",
                                "sample": f"{done_steps} Shu RK4 steps of the 1200x300 double Mach case ({closure} "
-                                         f"closure, no Z fallback, HLLC) from the step-{max(1, n_sr // 2)} state, "
+                                         f"closure, WENO7-Z fallback, HLLC) from the step-{max(1, n_sr // 2)} state, "
                                          f"OpenMP C oracle, {el:.1f} s",
                                "cpu_model": cpu_model_name(), "host_cpus": os.cpu_count()}
         out["gpu_vs_cpu"] = value / cpu_rate

@@ -56,8 +56,12 @@ enum {
 /* instrumentation flags (wsr_euler2d_config.flags) */
 enum {
     WSR_FLAG_COUNT_CLOSURE = 1, /* count reconstructions and closure (MLP) evaluations into the workspace */
-    WSR_FLAG_Z_FALLBACK = 2,    /* WENO7-SR: a closure ratio that is not finite or outside [0, 1] switches that
-                                   reconstruction to WENO7-Z (no reference counterpart) */
+    WSR_FLAG_Z_FALLBACK = 2,    /* WENO7-SR: per reconstructed value, the closure output is rejected and the
+                                   value reconstructed with WENO7-Z weights when a ratio is not finite or outside
+                                   [0, 1], or when the candidate the closure weights most (argmax d_k/(beta_k +
+                                   1e-12)^2) has a WENO7-Z indicator above 50 x the smallest one + 240 h^3 (it
+                                   crosses a discontinuity).  Counted in counter 6 (wsr_euler2d_counters).  No
+                                   reference counterpart. */
     WSR_FLAG_Z7_UNIT_GAUSS_H = 4 /* WENO7-Z: transverse Gauss-point weights with h = 1 (epsilon = 240) instead of
                                    the transverse spacing.  The reference's source passes the spacing
                                    (weno_z_borges_p2_results/weno7_point_rk4_z.py:473-500); its recorded 2-D
@@ -131,11 +135,15 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
                                  void *stream, double *sweep_ms, int32_t *sweep_launches);
 
 /* Closure counters accumulated since loop_init when WSR_FLAG_COUNT_CLOSURE is set;
- * copies the first n (1..6) of {normal reconstructions, of which closure
+ * copies the first n (1..7) of {normal reconstructions, of which closure
  * evaluated, Gauss-pair reconstructions, of which closure evaluated, lanes of
  * the waves that ran the normal closure, lanes of the waves that ran the Gauss
- * closure} to out (synchronises the stream).  No counterpart in the reference:
- * an instrument for the closure fraction that the roofline figures use. */
+ * closure, WENO7 values that fell back to WENO7-Z} to out (synchronises the
+ * stream).  The last one is collected whenever WSR_FLAG_Z_FALLBACK is set: each
+ * used normal face value and each Gauss-point value on a grid face counts once.
+ * wsr_euler7_rhs / wsr_euler7_step zero the counters at their start.  No
+ * counterpart in the reference: an instrument for the closure fraction that the
+ * roofline figures use and for the fallback rate. */
 int wsr_euler2d_counters(void *workspace, uint64_t *out, int32_t n, void *stream);
 
 /* ------------------------------------------------------------------------ */

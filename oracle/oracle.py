@@ -325,6 +325,10 @@ def _lib7():
         L.oracle_euler7_step.argtypes = [_P, _I, _I, _D, _D, _D, _D, _D, _I, _I, _I, _I, _P]
         L.oracle_euler7_run.restype = _I
         L.oracle_euler7_run.argtypes = [_P, _I, _I, _D, _D, _D, _D, _D, _I, _I, _I, _I, _P, _I, _I, _P, _P, _P]
+        L.oracle_guard7.restype = _I
+        L.oracle_guard7.argtypes = [_P, _I, _I, _I, _P, _D]
+        L.oracle_euler7_fallbacks.restype = ctypes.c_longlong
+        L.oracle_euler7_fallbacks.argtypes = []
         L._w7_ready = True
     return L
 
@@ -409,13 +413,29 @@ def riemann7(ul, ur, direction, c, gamma=1.4, riemann="hllc"):
     return out
 
 
+def _eno7(eno_cutoff: bool, z_fallback: bool) -> int:
+    """The oracle's WENO7 closure flags: bit 0 ENO cutoff, bit 1 WENO7-Z fallback."""
+    return (1 if eno_cutoff else 0) | (2 if z_fallback else 0)
+
+
+def guard7(q, gauss, lr, scheme, mlp, h) -> bool:
+    """The WENO7-Z fallback guard's decision for one stencil (SR closures)."""
+    return bool(_lib7().oracle_guard7(_ptr(_f64(q)), int(gauss), int(lr), SCHEMES[scheme],
+                                      _ptr(pack_mlp7(mlp, scheme)), float(h)))
+
+
+def fallbacks7() -> int:
+    """Values that fell back to WENO7-Z in the last rhs7_2d / step7_2d / run7_2d call."""
+    return int(_lib7().oracle_euler7_fallbacks())
+
+
 def rhs7_2d(u, nx, ny, dx, dy, gamma, dt, t=0.0, scheme="js", riemann="hllc", boundary="periodic",
-            reverse=False, eno_cutoff=False, mlp=None):
+            reverse=False, eno_cutoff=False, mlp=None, z_fallback=False):
     """One WENO7 right-hand side L(u) (ghost 4); the interior of the result."""
     u = _f64(u)
     rhs = np.zeros_like(u)
     rc = _lib7().oracle_euler7_rhs(_ptr(u), _ptr(rhs), nx, ny, dx, dy, gamma, dt, t, SCHEMES[scheme],
-                                   RIEMANN7[riemann], BOUNDARY[boundary], int(reverse), int(eno_cutoff),
+                                   RIEMANN7[riemann], BOUNDARY[boundary], int(reverse), _eno7(eno_cutoff, z_fallback),
                                    _ptr(pack_mlp7(mlp, scheme)))
     if rc:
         raise RuntimeError(f"oracle_euler7_rhs failed: {rc}")
@@ -423,17 +443,17 @@ def rhs7_2d(u, nx, ny, dx, dy, gamma, dt, t=0.0, scheme="js", riemann="hllc", bo
 
 
 def step7_2d(u, nx, ny, dx, dy, gamma, dt, t=0.0, scheme="js", riemann="hllc", boundary="periodic",
-             eno_cutoff=False, mlp=None):
+             eno_cutoff=False, mlp=None, z_fallback=False):
     u = np.array(u, dtype=np.float64, order="C")
     rc = _lib7().oracle_euler7_step(_ptr(u), nx, ny, dx, dy, gamma, dt, t, SCHEMES[scheme], RIEMANN7[riemann],
-                                    BOUNDARY[boundary], int(eno_cutoff), _ptr(pack_mlp7(mlp, scheme)))
+                                    BOUNDARY[boundary], _eno7(eno_cutoff, z_fallback), _ptr(pack_mlp7(mlp, scheme)))
     if rc:
         raise RuntimeError(f"oracle_euler7_step failed: {rc}")
     return u
 
 
 def run7_2d(u, nx, ny, dx, dy, gamma, cfl, t_end, scheme="js", riemann="hllc", boundary="periodic",
-            eno_cutoff=False, mlp=None, mode="point", max_steps=10_000_000):
+            eno_cutoff=False, mlp=None, mode="point", max_steps=10_000_000, z_fallback=False):
     """WENO7 + Shu RK4 time loop; mode "point" = point_rk4.run_from_initial,
     "double-mach" = weno7_double_mach.run_to_time.  Returns (u, dts, steps, t)."""
     u = np.array(u, dtype=np.float64, order="C")
@@ -442,7 +462,7 @@ def run7_2d(u, nx, ny, dx, dy, gamma, cfl, t_end, scheme="js", riemann="hllc", b
     steps = ctypes.c_int(0)
     t = ctypes.c_double(0.0)
     rc = _lib7().oracle_euler7_run(_ptr(u), nx, ny, dx, dy, gamma, cfl, t_end, SCHEMES[scheme], RIEMANN7[riemann],
-                                   BOUNDARY[boundary], int(eno_cutoff), _ptr(pack_mlp7(mlp, scheme)),
+                                   BOUNDARY[boundary], _eno7(eno_cutoff, z_fallback), _ptr(pack_mlp7(mlp, scheme)),
                                    0 if mode == "point" else 1, min(max_steps, cap), _ptr(dts),
                                    ctypes.byref(steps), ctypes.byref(t))
     if rc:

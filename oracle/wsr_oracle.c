@@ -1561,12 +1561,41 @@ static void w7_provider_beta(const double v[7], int side, int scheme, const void
     for (int k = 0; k < 4; ++k) beta[k] = side == 0 ? b[k] : b[3 - k];
 }
 
-/* WH7:741-776 weno7_lr_scalar (JS), EXT:92-120 external beta (SR), Z7:780-809 (Z) */
-static double w7_scalar_2d(const double q[7], int gauss, int lr, int scheme, const void *mlp, int eno, double h) {
+/* Guard of the neural closure (no reference counterpart; the per-value WENO-Z
+ * fallback of the double-Mach WENO7-SR workload): the closure output is rejected
+ * when a ratio beta_k / 4 is not finite or outside [0, 1], or when the candidate it
+ * weights most, k* = argmax d_k / (beta_k + 1e-12)^2 (first on ties), has a WENO7-Z
+ * indicator bz[k*] > 50 min_k bz_k + 240 h^3: the closure would let a candidate that
+ * crosses a discontinuity dominate. */
+#define W7_GUARD_RATIO 50.0
+static int w7_guard_trips(const double d[4], const double beta[4], const double bz[4], double h) {
+    int invalid = 0;
+    for (int k = 0; k < 4; ++k) {
+        double r = 0.25 * beta[k];
+        if (!(r >= 0.0 && r <= 1.0)) invalid = 1;
+    }
+    int ks = 0;
+    double best = -1.0;
+    for (int k = 0; k < 4; ++k) {
+        double ib = 1.0 / (beta[k] + 1.0e-12);
+        double a = d[k] * ib * ib;
+        if (a > best) { best = a; ks = k; }
+    }
+    double bmin = dmin(dmin(bz[0], bz[1]), dmin(bz[2], bz[3]));
+    return invalid || bz[ks] > W7_GUARD_RATIO * bmin + 240.0 * h * h * h;
+}
+
+/* WH7:741-776 weno7_lr_scalar (JS), EXT:92-120 external beta (SR), Z7:780-809 (Z).
+ * eno: bit 0 = ENO cutoff (EXT:46-62), bit 1 = test the SR closure's output with
+ * w7_guard_trips; *trip (if given) is set to 1 when the guard rejects it (the value
+ * returned is still the closure's: the caller replaces the whole state). */
+static double w7_scalar_2d(const double q[7], int gauss, int lr, int scheme, const void *mlp, int eno, double h,
+                           int *trip) {
     const int head = w7_head(gauss, lr);
     double s[4], d[4];
     w7_candidates(q, head, s);
     w7_optimal(head, d);
+    if (trip) *trip = 0;
     if (scheme == SCHEME_JS) {
         double b[4], a[4];
         w7_beta_poly(q, b);
@@ -1587,14 +1616,24 @@ static double w7_scalar_2d(const double q[7], int gauss, int lr, int scheme, con
         return (a[0] * s[0] + a[1] * s[1] + a[2] * s[2] + a[3] * s[3]) / asum;
     }
     double beta[4], a[4], ww[4];
-    w7_provider_beta(q, lr == 1 ? 0 : 1, scheme, mlp, beta);
+    const int side = lr == 1 ? 0 : 1;
+    w7_provider_beta(q, side, scheme, mlp, beta);
+    if ((eno & 2) && trip) {
+        double qs[7];
+        for (int m = 0; m < 7; ++m) qs[m] = side == 0 ? q[m] : q[6 - m];
+        if (!w7_plateau(qs)) {   /* the provider's plateau test, on its stencil orientation */
+            double bz[4];
+            w7_beta_sos(q, bz);
+            *trip = w7_guard_trips(d, beta, bz, h);
+        }
+    }
     for (int k = 0; k < 4; ++k) {
         double ib = 1.0 / (beta[k] + 1.0e-12);
         a[k] = d[k] * ib * ib;
     }
     double asum = a[0] + a[1] + a[2] + a[3];
     for (int k = 0; k < 4; ++k) ww[k] = a[k] / dmax(asum, 1.0e-300);
-    if (eno) {   /* EXT:46-62 */
+    if (eno & 1) {   /* EXT:46-62 */
         double psi[4], cut = 0.0;
         for (int k = 0; k < 4; ++k) psi[k] = ww[k] <= 4.0e-7 ? 0.0 : 1.0;
         cut = psi[0] * ww[0] + psi[1] * ww[1] + psi[2] * ww[2] + psi[3] * ww[3];
@@ -1653,23 +1692,51 @@ static vec4 w7_riemann(vec4 ul, vec4 ur, int direction, double c, double gamma, 
 }
 
 typedef struct {
-    int scheme, eno_cutoff, riemann;
+    int scheme, eno_cutoff, riemann;   /* eno_cutoff: bit 0 ENO cutoff, bit 1 WENO7-Z fallback */
     const void *mlp;
     double gamma;
+    int count;                         /* count fallbacks (the non-reversed right-hand sides) */
 } closure7_t;
+
+/* states that fell back to WENO7-Z since the last public WENO7 call: each normal
+ * face state (face, side) that a Gauss stencil uses and each Gauss-point state
+ * (face, side, point) counts once */
+static long long w7_fallbacks = 0;
+static void w7_count(int fell) {
+    if (fell) {
+#pragma omp atomic
+        w7_fallbacks += 1;
+    }
+}
+
+/* the four components of one reconstructed state: the closure's values, or, with
+ * the fallback enabled and the guard rejecting any component, WENO7-Z for all four
+ * (mixing the two per component can leave an unphysical state).  Returns 1 when the
+ * state fell back. */
+static int w7_state_2d(const double q[4][7], int gauss, int lr, const closure7_t *cl, double h, double out[4]) {
+    int any = 0;
+    for (int k = 0; k < 4; ++k) {
+        int trip = 0;
+        out[k] = w7_scalar_2d(q[k], gauss, lr, cl->scheme, cl->mlp, cl->eno_cutoff, h, &trip);
+        any |= trip;
+    }
+    if (!any) return 0;
+    for (int k = 0; k < 4; ++k) out[k] = w7_scalar_2d(q[k], gauss, lr, SCHEME_Z, NULL, 0, h, NULL);
+    return 1;
+}
 
 /* WH7:835-853 weno7_lr_vec_characteristic (EXT:168-204 for external beta):
  * lr 1 uses Roe(q3, q4), lr 2 Roe(q2, q3) */
-static vec4 w7_char_face(const vec4 qs[7], int lr, int direction, const closure7_t *cl, double h) {
+static vec4 w7_char_face(const vec4 qs[7], int lr, int direction, const closure7_t *cl, double h, int used) {
     vec4 roe = lr == 1 ? roe_average_state(qs[3], qs[4], cl->gamma) : roe_average_state(qs[2], qs[3], cl->gamma);
     vec4 c[7];
     for (int m = 0; m < 7; ++m) c[m] = con_to_char(qs[m], roe, direction, cl->gamma);
     vec4 face;
-    for (int k = 0; k < 4; ++k) {
-        double q[7];
-        for (int m = 0; m < 7; ++m) q[m] = c[m].v[k];
-        face.v[k] = w7_scalar_2d(q, 0, lr, cl->scheme, cl->mlp, cl->eno_cutoff, h);
-    }
+    double q[4][7];
+    for (int k = 0; k < 4; ++k)
+        for (int m = 0; m < 7; ++m) q[k][m] = c[m].v[k];
+    int fell = w7_state_2d(q, 0, lr, cl, h, face.v);
+    if (cl->count && used) w7_count(fell);
     return char_to_con(face, roe, direction, cl->gamma);
 }
 
@@ -1686,7 +1753,11 @@ static void w7_fill_boundary(const double *src, double *dst, int nx, int ny, int
  * states), P7:286-302 rhs_from_flux; one RHS evaluation L(src) into rhs (the
  * interior of rhs only).  work: 5 padded arrays + the four flux arrays. */
 static void w7_rhs_2d(const double *src, double *rhs, int nx, int ny, double dx, double dy, double dt, double time,
-                      int boundary, int reverse, const closure7_t *cl, double *work) {
+                      int boundary, int reverse, const closure7_t *cl_in, double *work) {
+    /* L~ reuses the reconstructions of L: only the non-reversed operator counts fallbacks */
+    closure7_t cl_local = *cl_in;
+    cl_local.count = !reverse;
+    const closure7_t *cl = &cl_local;
     const int gc = 4, nxt = nx + 2 * gc, nyt = ny + 2 * gc;
     const size_t np = (size_t)nxt * nyt * 4;
     double *bc = work, *xl = work + np, *xr = work + 2 * np, *yl = work + 3 * np, *yr = work + 4 * np;
@@ -1698,16 +1769,17 @@ static void w7_rhs_2d(const double *src, double *rhs, int nx, int ny, double dx,
         for (int i = 0; i < nx + 2; ++i) {
             vec4 qs[7];
             for (int m = 0; m < 7; ++m) qs[m] = load4(bc, j, i + m, nxt);
-            store4(xl, j, i, nxt, w7_char_face(qs, 2, 1, cl, dx));
-            store4(xr, j, i, nxt, w7_char_face(qs, 1, 1, cl, dx));
+            /* x_l at position i is the right state of face i - 1, x_r the left state of face i */
+            store4(xl, j, i, nxt, w7_char_face(qs, 2, 1, cl, dx, i >= 1));
+            store4(xr, j, i, nxt, w7_char_face(qs, 1, 1, cl, dx, i <= nx));
         }
 #pragma omp parallel for schedule(dynamic, 2)
     for (int j = 0; j < ny + 2; ++j)
         for (int i = 1; i < nx + 7; ++i) {
             vec4 qs[7];
             for (int m = 0; m < 7; ++m) qs[m] = load4(bc, j + m, i, nxt);
-            store4(yl, j, i, nxt, w7_char_face(qs, 2, 2, cl, dy));
-            store4(yr, j, i, nxt, w7_char_face(qs, 1, 2, cl, dy));
+            store4(yl, j, i, nxt, w7_char_face(qs, 2, 2, cl, dy, j >= 1));
+            store4(yr, j, i, nxt, w7_char_face(qs, 1, 2, cl, dy, j <= ny));
         }
     for (int loca = 1; loca <= 2; ++loca) {
         double *fx = loca == 1 ? fx1 : fx2, *fy = loca == 1 ? fy1 : fy2;
@@ -1715,15 +1787,15 @@ static void w7_rhs_2d(const double *src, double *rhs, int nx, int ny, double dx,
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx + 1; ++i) {
                 vec4 ul, ur;
-                for (int k = 0; k < 4; ++k) {
-                    double ql[7], qr[7];
+                double ql[4][7], qr[4][7];
+                for (int k = 0; k < 4; ++k)
                     for (int m = 0; m < 7; ++m) {
-                        ql[m] = xr[IDX(j + 1 + m, i, k, nxt)];
-                        qr[m] = xl[IDX(j + 1 + m, i + 1, k, nxt)];
+                        ql[k][m] = xr[IDX(j + 1 + m, i, k, nxt)];
+                        qr[k][m] = xl[IDX(j + 1 + m, i + 1, k, nxt)];
                     }
-                    ul.v[k] = w7_scalar_2d(ql, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dy);
-                    ur.v[k] = w7_scalar_2d(qr, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dy);
-                }
+                int fl = w7_state_2d(ql, 1, loca, cl, dy, ul.v);
+                int fr = w7_state_2d(qr, 1, loca, cl, dy, ur.v);
+                if (cl->count) { w7_count(fl); w7_count(fr); }
                 vec4 f = reverse ? w7_riemann(ur, ul, 1, dt / dx, cl->gamma, cl->riemann)
                                  : w7_riemann(ul, ur, 1, dt / dx, cl->gamma, cl->riemann);
                 for (int k = 0; k < 4; ++k) fx[((size_t)j * (nx + 1) + i) * 4 + k] = f.v[k];
@@ -1732,15 +1804,15 @@ static void w7_rhs_2d(const double *src, double *rhs, int nx, int ny, double dx,
         for (int j = 0; j < ny + 1; ++j)
             for (int i = 0; i < nx; ++i) {
                 vec4 ul, ur;
-                for (int k = 0; k < 4; ++k) {
-                    double ql[7], qr[7];
+                double ql[4][7], qr[4][7];
+                for (int k = 0; k < 4; ++k)
                     for (int m = 0; m < 7; ++m) {
-                        ql[m] = yr[IDX(j, i + 1 + m, k, nxt)];
-                        qr[m] = yl[IDX(j + 1, i + 1 + m, k, nxt)];
+                        ql[k][m] = yr[IDX(j, i + 1 + m, k, nxt)];
+                        qr[k][m] = yl[IDX(j + 1, i + 1 + m, k, nxt)];
                     }
-                    ul.v[k] = w7_scalar_2d(ql, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dx);
-                    ur.v[k] = w7_scalar_2d(qr, 1, loca, cl->scheme, cl->mlp, cl->eno_cutoff, dx);
-                }
+                int fl = w7_state_2d(ql, 1, loca, cl, dx, ul.v);
+                int fr = w7_state_2d(qr, 1, loca, cl, dx, ur.v);
+                if (cl->count) { w7_count(fl); w7_count(fr); }
                 vec4 f = reverse ? w7_riemann(ur, ul, 2, dt / dy, cl->gamma, cl->riemann)
                                  : w7_riemann(ul, ur, 2, dt / dy, cl->gamma, cl->riemann);
                 for (int k = 0; k < 4; ++k) fy[((size_t)j * nx + i) * 4 + k] = f.v[k];
@@ -1771,7 +1843,8 @@ int oracle_euler7_rhs(const double *u, double *rhs, int nx, int ny, double dx, d
     if ((scheme == SCHEME_SR64 || scheme == SCHEME_SR32) && !mlp) return -2;
     double *work = (double *)malloc(sizeof(double) * oracle_euler7_work_doubles(nx, ny));
     if (!work) return -3;
-    closure7_t cl = {scheme, eno_cutoff, riemann, mlp, gamma};
+    closure7_t cl = {scheme, eno_cutoff, riemann, mlp, gamma, 1};
+    w7_fallbacks = 0;
     w7_rhs_2d(u, rhs, nx, ny, dx, dy, dt, time, boundary, reverse, &cl, work);
     free(work);
     return 0;
@@ -1779,7 +1852,7 @@ int oracle_euler7_rhs(const double *u, double *rhs, int nx, int ny, double dx, d
 
 /* P7:314-404 / DM7:354-431 one Shu RK4 step of the padded state u (in place);
  * boundary times t, t, t + dt/2, t + dt/2, t + dt/2, t + dt as in DM7 */
-int oracle_euler7_step(double *u, int nx, int ny, double dx, double dy, double gamma, double dt, double t,
+static int euler7_step(double *u, int nx, int ny, double dx, double dy, double gamma, double dt, double t,
                        int scheme, int riemann, int boundary, int eno_cutoff, const void *mlp) {
     if ((scheme == SCHEME_SR64 || scheme == SCHEME_SR32) && !mlp) return -2;
     const size_t np = (size_t)(nx + 8) * (ny + 8) * 4;
@@ -1787,7 +1860,7 @@ int oracle_euler7_step(double *u, int nx, int ny, double dx, double dy, double g
     if (!work) return -3;
     double *u1 = work + oracle_euler7_work_doubles(nx, ny), *u2 = u1 + np, *u3 = u2 + np;
     double *r0 = u3 + np, *rt0 = r0 + np, *r1 = rt0 + np, *rt1 = r1 + np, *r2 = rt1 + np, *r3 = r2 + np;
-    closure7_t cl = {scheme, eno_cutoff, riemann, mlp, gamma};
+    closure7_t cl = {scheme, eno_cutoff, riemann, mlp, gamma, 1};
     w7_rhs_2d(u, r0, nx, ny, dx, dy, dt, t, boundary, 0, &cl, work);
     for (size_t e = 0; e < np; ++e) u1[e] = u[e] + 0.5 * dt * r0[e];
     w7_rhs_2d(u, rt0, nx, ny, dx, dy, dt, t, boundary, 1, &cl, work);
@@ -1808,6 +1881,15 @@ int oracle_euler7_step(double *u, int nx, int ny, double dx, double dy, double g
     return 0;
 }
 
+int oracle_euler7_step(double *u, int nx, int ny, double dx, double dy, double gamma, double dt, double t,
+                       int scheme, int riemann, int boundary, int eno_cutoff, const void *mlp) {
+    w7_fallbacks = 0;
+    return euler7_step(u, nx, ny, dx, dy, gamma, dt, t, scheme, riemann, boundary, eno_cutoff, mlp);
+}
+
+/* states that fell back to WENO7-Z in the last oracle_euler7_rhs / _step / _run call */
+long long oracle_euler7_fallbacks(void) { return w7_fallbacks; }
+
 /* Time loops.  mode 0 = P7:557-575 run_from_initial (dt clipped when t + dt
  * passes t_end), mode 1 = DM7:453-470 run_to_time (stops at t_end - 1e-14,
  * dt = min(dt_cfl, t_end - t), a non-finite dt ends the loop). */
@@ -1816,6 +1898,7 @@ int oracle_euler7_run(double *u, int nx, int ny, double dx, double dy, double ga
                       int max_steps, double *dts, int *steps_out, double *t_out) {
     double t = 0.0;
     int step = 0;
+    w7_fallbacks = 0;
     while (step < max_steps) {
         if (mode == 0 ? !(t < t_end) : !(t < t_end - 1.0e-14)) break;
         double dt = oracle_euler2d_dt(u, nx, ny, 4, dx, dy, gamma, cfl);
@@ -1827,7 +1910,7 @@ int oracle_euler7_run(double *u, int nx, int ny, double dx, double dy, double ga
             dt = rem < dt ? rem : dt;
             if (!isfinite(dt) || dt <= 0.0) break;
         }
-        int rc = oracle_euler7_step(u, nx, ny, dx, dy, gamma, dt, t, scheme, riemann, boundary, eno_cutoff, mlp);
+        int rc = euler7_step(u, nx, ny, dx, dy, gamma, dt, t, scheme, riemann, boundary, eno_cutoff, mlp);
         if (rc) return rc;
         if (dts) dts[step] = dt;
         t += dt;
@@ -1841,7 +1924,13 @@ int oracle_euler7_run(double *u, int nx, int ny, double dx, double dy, double ga
 void oracle_beta7_poly(const double *q, double *b) { w7_beta_poly(q, b); }
 void oracle_beta7_sos(const double *q, double *b) { w7_beta_sos(q, b); }
 double oracle_scalar7_2d(const double *q, int gauss, int lr, int scheme, const void *mlp, int eno, double h) {
-    return w7_scalar_2d(q, gauss, lr, scheme, mlp, eno, h);
+    return w7_scalar_2d(q, gauss, lr, scheme, mlp, eno, h, NULL);
+}
+/* 1 when the WENO7-Z fallback guard rejects the SR closure's value of this stencil */
+int oracle_guard7(const double *q, int gauss, int lr, int scheme, const void *mlp, double h) {
+    int trip = 0;
+    (void)w7_scalar_2d(q, gauss, lr, scheme, mlp, 2, h, &trip);
+    return trip;
 }
 void oracle_riemann7(const double *ul, const double *ur, int direction, double c, double gamma, int riemann,
                      double *out) {

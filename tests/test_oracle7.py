@@ -176,3 +176,84 @@ def test_weno7_uniform_flow_is_preserved(oracle, mlp7):
             u = oracle.step7_2d(u0, n, n, 0.1, 0.1, 1.4, 0.01, scheme=scheme, boundary=boundary,
                                 mlp=mlp7 if scheme == "sr64" else None)
             assert np.max(np.abs(u[4:-4, 4:-4] - u0[4:-4, 4:-4])) <= 1e-14
+
+
+# --------------------------------------------------------------------------
+# WENO7-Z fallback of the SR closure (the double-Mach WENO7-SR workload's
+# per-state fallback; no reference counterpart).  The guard restated
+# independently: the candidate the closure weights most must not carry a
+# WENO7-Z indicator above 50 x the smallest one + 240 h^3.
+def _guard_restated(oracle, q, gauss, lr, scheme, mlp, h):
+    head = (4 if lr == 1 else 3) if gauss else lr
+    q = np.asarray(q, dtype=np.float64)
+    qs = q if lr == 1 else q[::-1]            # the lr = 2 side evaluates the mirrored stencil
+    if oracle.plateau7(qs):
+        return False
+    r = np.asarray(oracle.mlp7(oracle.features7(qs), scheme, mlp), dtype=np.float64).reshape(4)
+    beta = 4.0 * (r if lr == 1 else r[::-1])
+    d = np.asarray(oracle.optimal7(head))
+    bz = np.asarray(oracle.beta7_sos(q))
+    invalid = not bool(np.all((beta / 4.0 >= 0.0) & (beta / 4.0 <= 1.0)))
+    dominant = int(np.argmax(d / (beta + 1.0e-12) ** 2))
+    return invalid or bool(bz[dominant] > 50.0 * bz.min() + 240.0 * h ** 3)
+
+
+def _guard_stencils(oracle, mlp7):
+    from weno_sr_publishing_amd.initial import make_double_mach_state
+    nx, ny = 60, 15
+    u, _, _, _ = oracle.run7_2d(make_double_mach_state(nx, ny), nx, ny, 4.0 / nx, 1.0 / ny, 1.4, 0.4, 0.2,
+                                scheme="sr64", riemann="hllc", boundary="double-mach", mlp=mlp7,
+                                mode="double-mach", max_steps=10)
+    rows = []
+    for comp in range(4):
+        a = u[..., comp]
+        rows.append(np.stack([a[:, m:a.shape[1] - 6 + m] for m in range(7)], -1).reshape(-1, 7))
+        rows.append(np.stack([a[m:a.shape[0] - 6 + m, :] for m in range(7)], -1).reshape(-1, 7))
+    rng = np.random.default_rng(7)
+    steps = np.where(np.arange(7)[None, :] >= rng.integers(1, 7, size=(200, 1)), 1.0, 0.0)
+    steps = steps * rng.uniform(0.5, 5.0, size=(200, 1)) + rng.normal(scale=0.02, size=(200, 7))
+    smooth = np.sin(0.3 * np.arange(7)[None, :] + rng.uniform(0, 6.3, size=(100, 1)))
+    return np.concatenate(rows[::2] + [steps, smooth])[::3]
+
+
+def test_z7_guard_matches_restatement(oracle, mlp7):
+    qs = _guard_stencils(oracle, mlp7)
+    trips = 0
+    for q in qs:
+        for gauss in (0, 1):
+            for lr in (1, 2):
+                got = oracle.guard7(q, gauss, lr, "sr64", mlp7, 1.0 / 15.0)
+                assert got == _guard_restated(oracle, q, gauss, lr, "sr64", mlp7, 1.0 / 15.0), (q, gauss, lr)
+                trips += got
+    # the guard must be able to fire on finite data, and must not fire everywhere
+    assert 0 < trips < 0.5 * 4 * len(qs)
+
+
+def test_z7_fallback_double_mach_oracle_run(oracle, mlp7):
+    """The fallback fires on the double-Mach flow and keeps it physical; without the
+    flag nothing is counted."""
+    from weno_sr_publishing_amd.initial import make_double_mach_state
+    nx, ny = 60, 15
+    u0 = make_double_mach_state(nx, ny)
+    kw = dict(scheme="sr32", riemann="hllc", boundary="double-mach", mlp=mlp7.as_precision("fp32"),
+              mode="double-mach", max_steps=25)
+    u, dts, steps, _ = oracle.run7_2d(u0, nx, ny, 4.0 / nx, 1.0 / ny, 1.4, 0.4, 0.2, z_fallback=True, **kw)
+    fb = oracle.fallbacks7()
+    assert steps == 25 and fb > 0
+    inner = u[4:-4, 4:-4]
+    assert np.isfinite(inner).all()
+    rho = inner[..., 0]
+    p = 0.4 * (inner[..., 3] - 0.5 * (inner[..., 1] ** 2 + inner[..., 2] ** 2) / rho)
+    assert rho.min() > 0 and p.min() > 0
+    oracle.run7_2d(u0, nx, ny, 4.0 / nx, 1.0 / ny, 1.4, 0.4, 0.2, z_fallback=False, **kw)
+    assert oracle.fallbacks7() == 0
+
+
+def test_z7_fallback_quiet_on_resolved_smooth_flow(oracle, mlp7):
+    """The periodic isentropic vortex at 50 x 50: no state falls back."""
+    from weno_sr_publishing_amd.initial import make_vortex_state
+    n = 50
+    u = make_vortex_state(n, n)
+    oracle.rhs7_2d(u, n, n, 20.0 / n, 20.0 / n, 1.4, 0.1, 0.0, scheme="sr64", boundary="periodic", mlp=mlp7,
+                   z_fallback=True)
+    assert oracle.fallbacks7() == 0

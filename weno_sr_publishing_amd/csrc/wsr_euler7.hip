@@ -116,6 +116,7 @@ struct SweepArgs7 {
     double dt_host;
     const Ctl *ctl;
     unsigned long long *counters;
+    unsigned long long *fallbacks;   // counter of values that fell back to WENO7-Z (with z_fallback)
     int eno;
     int z_fallback;
     int unit_gauss_h;   // WSR_FLAG_Z7_UNIT_GAUSS_H: transverse weights with h = 1
@@ -190,6 +191,7 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
     const int k = gi & 3;
     const int slot = lane >> 3;
     LaneCounts cnt;
+    unsigned nfell = 0;
 
     // phase 1b: characteristic projection, WENO7 face value, back-projection
     for (int base = 0; base < N1; base += SBLOCK / 8) {
@@ -207,10 +209,24 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
             const double *q = T + ((fl + s + m) * NT1 + tau) * 4;
             c[m] = (l[0] * q[0] + l[1] * q[1] + l[2] * q[2] + l[3] * q[3]) * scale;
         }
-        bool closure, fell;
-        double val = s == 0 ? recon7_normal<SCHEME, 1>(c, mlp, A.eno, h_n, zf, closure, fell)
-                            : recon7_normal<SCHEME, 2>(c, mlp, A.eno, h_n, zf, closure, fell);
+        bool closure, trip;
+        double val = s == 0 ? recon7_normal<SCHEME, 1>(c, mlp, A.eno, h_n, zf, closure, trip)
+                            : recon7_normal<SCHEME, 2>(c, mlp, A.eno, h_n, zf, closure, trip);
         if (A.counters) cnt.add(0, 4, true, closure);
+        if (zf) {
+            // the four lanes of one side hold the four characteristic components of one state
+            const unsigned long long tb = __ballot(trip);
+            if ((tb >> (lane & 60)) & 0xFull) {   // the group's bits in the wave mask
+                val = s == 0 ? z7_value<1>(c, h_n) : z7_value<2>(c, h_n);
+                // count each used state once: faces of the grid, and the face-lines this
+                // tile owns (line L = t0 - 3 + tau in [-3, ntrans + 3); a halo line belongs
+                // to the tile whose core holds L clamped to the grid)
+                const int L = t0 - 3 + tau;
+                const int Lc = L < 0 ? 0 : (L > ntrans - 1 ? ntrans - 1 : L);
+                const bool owned = (f0 + fl) < nfaces && L < ntrans + 3 && Lc >= t0 && Lc < t0 + TT;
+                nfell += (owned && k == 0) ? 1u : 0u;
+            }
+        }
         const int qb = lane & ~3;
         double cf0 = __shfl(val, qb + 0), cf1 = __shfl(val, qb + 1);
         double cf2 = __shfl(val, qb + 2), cf3 = __shfl(val, qb + 3);
@@ -234,9 +250,19 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
 #pragma unroll
             for (int m = 0; m < 7; ++m) q[m] = FV[((tl + m) * TN + fl) * 8 + gi];
             double g1, g2;
-            bool closure, fell;
-            recon7_gauss<SCHEME>(q, mlp, A.eno, h_t, zf, g1, g2, closure, fell);
+            bool closure, trip1, trip2;
+            recon7_gauss<SCHEME>(q, mlp, A.eno, h_t, zf, g1, g2, closure, trip1, trip2);
             if (A.counters) cnt.add(2, 5, (f0 + fl) < nfaces && (t0 + tl) < ntrans, closure);
+            if (zf) {
+                // lanes of a group of four hold one side's four conservative components
+                // (items past the tile skip whole groups of eight lanes)
+                const unsigned long long b1 = __ballot(trip1), b2 = __ballot(trip2);
+                const bool z1 = (b1 >> (lane & 60)) & 0xFull, z2 = (b2 >> (lane & 60)) & 0xFull;
+                if (z1) g1 = z7_value<4>(q, h_t);
+                if (z2) g2 = z7_value<3>(q, h_t);
+                // states of faces inside the grid (the tile's edge lines past it are not counted)
+                if (k == 0 && (f0 + fl) < nfaces && (t0 + tl) < ntrans) nfell += (z1 ? 1u : 0u) + (z2 ? 1u : 0u);
+            }
             GV[ci * 16 + gi] = g1;
             GV[ci * 16 + 8 + gi] = g2;
         }
@@ -273,6 +299,10 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
         __syncthreads();
     }
     if (A.counters) cnt.flush(A.counters);
+    if (A.fallbacks) {
+        for (int off = 32; off > 0; off >>= 1) nfell += __shfl_down(nfell, off);
+        if ((threadIdx.x & 63) == 0 && nfell) atomicAdd(A.fallbacks, (unsigned long long)nfell);
+    }
 }
 
 // Both sweeps of a right-hand side in one launch: workgroups [0, gx*tx) take the x
@@ -455,6 +485,12 @@ __global__ __launch_bounds__(BLOCK) void k7_max_speed(const double *__restrict__
         sp = cell_speed(q, g.gamma);
     }
     reduce_speed<BLOCK>(sp, active, ctl);
+}
+
+// counters cover one wsr_euler7_rhs / wsr_euler7_step call (a kernel, not a memset:
+// the call may be captured into a graph)
+__global__ void k7_zero_counters(Ctl *ctl) {
+    if (threadIdx.x < NCOUNT) ctl->counters[threadIdx.x] = 0ull;
 }
 
 __global__ void k7_loop_init(Ctl *ctl, double t_end, double cfl_h, int max_steps) {
@@ -641,6 +677,7 @@ static void enqueue_rhs7(const wsr_euler7_config *cfg, const Workspace7 &w, cons
     A.z_fallback = (cfg->flags & WSR_FLAG_Z_FALLBACK) ? 1 : 0;
     A.unit_gauss_h = (cfg->flags & WSR_FLAG_Z7_UNIT_GAUSS_H) ? 1 : 0;
     A.counters = (cfg->flags & WSR_FLAG_COUNT_CLOSURE) ? w.ctl->counters : nullptr;
+    A.fallbacks = A.z_fallback ? w.ctl->counters + 6 : nullptr;
     if (tm) tm->begin(st);
     launch7(cfg, w.bc, w.Fx, both ? w.Fxt : nullptr, w.Fy, both ? w.Fyt : nullptr, A, w.mlp, st);
     if (tm) tm->end(both ? 0 : 1, st);
@@ -690,6 +727,7 @@ int wsr_euler7_rhs(const wsr_euler7_config *cfg, const double *u, double *rhs, d
     hipStream_t st = (hipStream_t)stream;
     Grid g = grid7(cfg);
     enqueue_params7(cfg, w, st);
+    hipLaunchKernelGGL(k7_zero_counters, dim3(1), dim3(64), 0, st, w.ctl);
     enqueue_rhs7(cfg, w, u, rhs_reverse != nullptr, dt, time, nullptr, 0.0, st);
     const int np = g.nxt * g.nyt;
     hipLaunchKernelGGL(k7_rhs_out, dim3((np + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, (const double *)w.Fx,
@@ -708,6 +746,7 @@ int wsr_euler7_step(const wsr_euler7_config *cfg, double *u, double dt, double t
     ws7_layout(cfg->nx, cfg->ny, &w, workspace);
     hipStream_t st = (hipStream_t)stream;
     enqueue_params7(cfg, w, st);
+    hipLaunchKernelGGL(k7_zero_counters, dim3(1), dim3(64), 0, st, w.ctl);
     enqueue_step7(cfg, w, u, dt, time, nullptr, nullptr, st);
     return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
 }

@@ -11,7 +11,9 @@
 
 namespace wsr {
 
-constexpr int NCOUNT = 6;
+// counters 0-5: closure evaluations (WSR_FLAG_COUNT_CLOSURE); 6: WENO7 values that fell
+// back to WENO7-Z (WSR_FLAG_Z_FALLBACK)
+constexpr int NCOUNT = 7;
 constexpr int SPEED_SLOTS = 16, SPEED_SLOT_STRIDE = 16;
 
 struct Ctl {
@@ -21,7 +23,8 @@ struct Ctl {
     // closure evaluations (WSR_FLAG_COUNT_CLOSURE): [0] normal reconstructions,
     // [1] of which evaluated the closure (not a plateau), [2] Gauss-pair
     // reconstructions, [3] of which evaluated the closure, [4] / [5] lanes of
-    // the waves that ran the closure code (normal / Gauss; 64 per wave pass)
+    // the waves that ran the closure code (normal / Gauss; 64 per wave pass);
+    // [6] WENO7 reconstructed values that fell back to WENO7-Z (WSR_FLAG_Z_FALLBACK)
     unsigned long long counters[NCOUNT];
     // per-block maxima of the fused speed reduction, spread over SPEED_SLOTS
     // addresses on separate 128-byte lines (one address would serialise every
@@ -49,7 +52,7 @@ __device__ __forceinline__ void dm_exact(double x, double y, double t, double ga
 
 // per-lane closure counts, summed over the wave and added once per wave at the end
 struct LaneCounts {
-    unsigned c[NCOUNT] = {0u, 0u, 0u, 0u, 0u, 0u};
+    unsigned c[NCOUNT] = {};
     __device__ __forceinline__ void add(int slot, int wave_slot, bool counted, bool evaluated) {
         c[slot] += counted ? 1u : 0u;
         c[slot + 1] += (counted && evaluated) ? 1u : 0u;

@@ -308,15 +308,52 @@ __device__ __forceinline__ double combine_z7(const double s[4], const double d[4
     return (a[0] * s[0] + a[1] * s[1] + a[2] * s[2] + a[3] * s[3]) / asum;
 }
 
+// Guard of the neural closure (WSR_FLAG_Z_FALLBACK).  The candidate the closure
+// weights most, k* = argmax d_k / (beta_k + 1e-12)^2, must not be one that the WENO7-Z
+// smoothness indicators bz mark as crossing a discontinuity:
+// bz[k*] > Z7_GUARD_RATIO min_k bz_k + 240 h^3 (the WENO7-Z epsilon) rejects the closure
+// output, and so does a ratio that is not finite or outside [0, 1] (`invalid`).  A state
+// (the four components of one face side, or of one Gauss point of a side) with a
+// rejected component is reconstructed with the WENO7-Z weights in all four components:
+// the kernels decide per group of four lanes.
+constexpr double Z7_GUARD_RATIO = 50.0;
+
+__device__ __forceinline__ bool z7_guard_trips(const double d[4], const double beta[4], const double bz[4],
+                                               double h, bool invalid) {
+    int ks = 0;
+    double best = -1.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double ib = 1.0 / (beta[k] + 1.0e-12);
+        double a = d[k] * ib * ib;
+        if (a > best) { best = a; ks = k; }
+    }
+    double bmin = dmin(dmin(bz[0], bz[1]), dmin(bz[2], bz[3]));
+    double bk = ks == 0 ? bz[0] : (ks == 1 ? bz[1] : (ks == 2 ? bz[2] : bz[3]));
+    return invalid || bk > Z7_GUARD_RATIO * bmin + 240.0 * h * h * h;
+}
+
+// WENO7-Z value of one component at head HEAD (the fallback)
+template <int HEAD>
+__device__ __forceinline__ double z7_value(const double q[7], double h) {
+    double s[4], d[4], bz[4];
+    candidates7<HEAD>(q, s);
+    optimal7<HEAD>(d);
+    beta7_sos(q, bz);
+    return combine_z7(s, d, bz, h);
+}
+
 // Normal face value of one characteristic component: lr 1 (HEAD 1) or lr 2 (HEAD 2).
+// With z_fallback, `trip` reports whether the guard rejects the closure's value (the
+// value returned is still the closure's).
 template <int SCHEME, int HEAD>
 __device__ __forceinline__ double recon7_normal(const double q[7], const typename Mlp7Sweep<SCHEME>::type &P,
-                                                int eno, double h, bool z_fallback, bool &closure, bool &fell_back) {
+                                                int eno, double h, bool z_fallback, bool &closure, bool &trip) {
     double s[4], d[4], b[4];
     candidates7<HEAD>(q, s);
     optimal7<HEAD>(d);
     closure = false;
-    fell_back = false;
+    trip = false;
     if constexpr (SCHEME == SCHEME_JS) {
         beta7_poly(q, b);
         return combine_js7(s, d, b);
@@ -326,10 +363,10 @@ __device__ __forceinline__ double recon7_normal(const double q[7], const typenam
     } else {
         bool invalid;
         closure = closure_beta7<SCHEME>(q, HEAD == 2, P, b, invalid);
-        if (z_fallback && invalid) {
-            fell_back = true;
-            beta7_sos(q, b);
-            return combine_z7(s, d, b, h);
+        if (z_fallback && closure) {
+            double bz[4];
+            beta7_sos(q, bz);
+            trip = z7_guard_trips(d, b, bz, h, invalid);
         }
         return combine_beta7(s, d, b, eno);
     }
@@ -338,17 +375,20 @@ __device__ __forceinline__ double recon7_normal(const double q[7], const typenam
 // Both transverse Gauss-point values (loca 1 = head 4, loca 2 = head 3) of one
 // conservative component; one closure evaluation serves both points (the
 // reference evaluates the mirrored stencil for loca 2, equal up to rounding).
+// With z_fallback the guard judges each Gauss point on its own (its optimal weights
+// pick the dominant candidate): trip1 / trip2 (the values are still the closure's).
 template <int SCHEME>
 __device__ __forceinline__ void recon7_gauss(const double q[7], const typename Mlp7Sweep<SCHEME>::type &P, int eno,
                                              double h, bool z_fallback, double &g1, double &g2, bool &closure,
-                                             bool &fell_back) {
+                                             bool &trip1, bool &trip2) {
     double s1[4], s2[4], d1[4], d2[4], b[4];
     candidates7<4>(q, s1);
     candidates7<3>(q, s2);
     optimal7<4>(d1);
     optimal7<3>(d2);
     closure = false;
-    fell_back = false;
+    trip1 = false;
+    trip2 = false;
     if constexpr (SCHEME == SCHEME_JS) {
         beta7_poly(q, b);
         g1 = combine_js7(s1, d1, b);
@@ -360,15 +400,14 @@ __device__ __forceinline__ void recon7_gauss(const double q[7], const typename M
     } else {
         bool invalid;
         closure = closure_beta7<SCHEME>(q, false, P, b, invalid);
-        if (z_fallback && invalid) {
-            fell_back = true;
-            beta7_sos(q, b);
-            g1 = combine_z7(s1, d1, b, h);
-            g2 = combine_z7(s2, d2, b, h);
-        } else {
-            g1 = combine_beta7(s1, d1, b, eno);
-            g2 = combine_beta7(s2, d2, b, eno);
+        if (z_fallback && closure) {
+            double bz[4];
+            beta7_sos(q, bz);
+            trip1 = z7_guard_trips(d1, b, bz, h, invalid);
+            trip2 = z7_guard_trips(d2, b, bz, h, invalid);
         }
+        g1 = combine_beta7(s1, d1, b, eno);
+        g2 = combine_beta7(s2, d2, b, eno);
     }
 }
 

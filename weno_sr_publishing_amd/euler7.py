@@ -191,10 +191,19 @@ class Euler7Solver:
             self.cfg.flags &= ~_lib.FLAG_COUNT_CLOSURE
 
     def counters(self) -> list[int]:
-        out = (ctypes.c_uint64 * 6)()
-        rc = self._lib.wsr_euler2d_counters(self.workspace.data_ptr(), out, 6, self._stream())
+        """[normal recons, closures, Gauss recons, closures, lanes of closure waves (normal,
+        Gauss), states that fell back to WENO7-Z]: the first six with set_counting(True),
+        the last whenever the solver was built with z_fallback."""
+        out = (ctypes.c_uint64 * 7)()
+        rc = self._lib.wsr_euler2d_counters(self.workspace.data_ptr(), out, 7, self._stream())
         _lib.check(rc, "wsr_euler2d_counters")
         return [int(v) for v in out]
+
+    def fallbacks(self) -> int:
+        """Reconstructed states that fell back to WENO7-Z since loop_init (or in the last
+        rhs / step call): each normal face state (face, side) and each Gauss-point state
+        (face, side, point) counts once."""
+        return self.counters()[6]
 
     def run(self, u: torch.Tensor, t_end: float, report_interval: int = 0,
             report: Callable[[int, float, float, dict], None] | None = None, check_health: bool = False,
