@@ -98,6 +98,21 @@ int wsr_euler2d_stage(const wsr_euler2d_config *cfg, const double *u_stage, cons
                       double *u_out, double dt, int32_t rk, double boundary_time,
                       void *workspace, size_t workspace_bytes, void *stream);
 
+/* Allocation-free variant of wsr_euler2d_stage.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and results
+ * as wsr_euler2d_stage (the same kernels), without the workspace: the stage works in a
+ * file-scope device array of WSR_NOALLOC_SCRATCH_BYTES (32 MiB; each GPU the library
+ * runs on holds it until the process ends, and its device address is looked up once
+ * per device).  Grids whose workspace (wsr_euler2d_workspace_bytes) exceeds it return
+ * WSR_E_WORKSPACE (400 x 400 needs 15.5 MB; up to about 570 x 570 fits);
+ * WSR_FLAG_COUNT_CLOSURE is refused (WSR_E_ARG).  The call allocates nothing.  Every call
+ * on one GPU shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race. */
+#define WSR_NOALLOC_SCRATCH_BYTES (32u << 20)
+int wsr_euler2d_stage_noalloc(const wsr_euler2d_config *cfg, const double *u_stage, const double *u0,
+                              double *u_out, double dt, int32_t rk, double boundary_time, void *stream);
+
 /* Maximum characteristic speed max(|u|+c, |v|+c) over the interior, reduced on
  * the device and returned to the host (synchronises the stream).  Replaces
  * compute_max_speed_kernel + np.max of warp_weno5_helpers.py:189-207,876-897;

@@ -17,7 +17,7 @@ ERRORS = {-1: "bad argument", -2: "SR scheme without MLP parameters", -3: "works
           -4: "HIP runtime error"}
 
 EXPORTS = (
-    "wsr_euler2d_workspace_bytes", "wsr_euler2d_stage", "wsr_euler2d_max_speed",
+    "wsr_euler2d_workspace_bytes", "wsr_euler2d_stage", "wsr_euler2d_stage_noalloc", "wsr_euler2d_max_speed",
     "wsr_euler2d_loop_init", "wsr_euler2d_loop_steps", "wsr_euler2d_loop_status", "wsr_version",
     "wsr_euler2d_loop_steps_timed", "wsr_euler2d_counters", "wsr_adv1d_rollout",
     "wsr_adv1d_train_forward", "wsr_adv1d_train_adjoint",
@@ -28,6 +28,7 @@ EXPORTS = (
 )
 FLAG_COUNT_CLOSURE = 1
 FLAG_Z_FALLBACK = 2
+NOALLOC_SCRATCH_BYTES = 32 << 20   # WSR_NOALLOC_SCRATCH_BYTES
 FLAG_Z7_UNIT_GAUSS_H = 4
 LOOP7 = {"point": 0, "double-mach": 1}
 
@@ -103,6 +104,8 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler2d_workspace_bytes.argtypes = [I32, I32, I32]
         L.wsr_euler2d_stage.restype = ctypes.c_int
         L.wsr_euler2d_stage.argtypes = [CFG, P, P, P, D, I32, D, P, SZ, P]
+        L.wsr_euler2d_stage_noalloc.restype = ctypes.c_int
+        L.wsr_euler2d_stage_noalloc.argtypes = [CFG, P, P, P, D, I32, D, P]
         L.wsr_euler2d_max_speed.restype = ctypes.c_int
         L.wsr_euler2d_max_speed.argtypes = [CFG, P, P, SZ, ctypes.POINTER(D), P]
         L.wsr_euler2d_loop_init.restype = ctypes.c_int

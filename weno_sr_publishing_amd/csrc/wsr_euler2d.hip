@@ -35,6 +35,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "../../include/wsr.h"
 #include "wsr_math.hpp"
 #include "wsr_loop.hpp"
@@ -633,6 +635,32 @@ static void enqueue_loop_ghosts(const wsr_euler2d_config *cfg, const Workspace &
                        (const Ctl *)w.ctl);
 }
 
+// ------------------------------------------------ allocation-free stage
+// File-scope device scratch for wsr_euler2d_stage_noalloc: the workspace of one stage
+// (control block, closure parameters, boundary-filled copy, both flux arrays).  Each
+// GPU the library runs on holds its own copy until the process ends.  In host code
+// `scratch` names a host-side stand-in, never the device memory: the device address
+// comes from hipGetSymbolAddress, once per device, and only that address is used.
+static __device__ __align__(256) unsigned char scratch[WSR_NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+static void *g_scratch_dev[MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// device address of `scratch` on the current device (looked up on the first call there)
+static int scratch_address(void **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return WSR_E_HIP;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return WSR_E_HIP;
+        g_scratch_dev[dev] = ptr;
+    }
+    *out = g_scratch_dev[dev];
+    return WSR_OK;
+}
+
 }  // namespace wsr
 
 using namespace wsr;
@@ -655,6 +683,27 @@ int wsr_euler2d_stage(const wsr_euler2d_config *cfg, const double *u_stage, cons
     Workspace w;
     ws_layout(cfg->nx, cfg->ny, &w, workspace);
     hipStream_t st = (hipStream_t)stream;
+    enqueue_params(cfg, w, st);
+    enqueue_stage(cfg, w, u_stage, u0, u_out, dt, rk, boundary_time, nullptr, 0, nullptr, st);
+    return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler2d_stage_noalloc(const wsr_euler2d_config *cfg, const double *u_stage, const double *u0,
+                              double *u_out, double dt, int32_t rk, double boundary_time, void *stream) {
+    int rc = check_cfg(cfg);
+    if (rc) return rc;
+    if (!u_stage || !u0 || !u_out || rk < 1 || rk > 3) return WSR_E_ARG;
+    if (cfg->flags & WSR_FLAG_COUNT_CLOSURE) return WSR_E_ARG;   // no caller-visible counters here
+    if (!all_aligned16(u_stage, u0, u_out)) return WSR_E_ARG;
+    if (ws_layout(cfg->nx, cfg->ny, nullptr, nullptr) > WSR_NOALLOC_SCRATCH_BYTES) return WSR_E_WORKSPACE;
+    void *base = nullptr;
+    rc = scratch_address(&base);
+    if (rc) return rc;
+    Workspace w;
+    ws_layout(cfg->nx, cfg->ny, &w, base);
+    hipStream_t st = (hipStream_t)stream;
+    // every scratch region the stage reads is written first in the same call
+    // (parameters, boundary-filled copy, every face of both flux arrays): nothing to clear
     enqueue_params(cfg, w, st);
     enqueue_stage(cfg, w, u_stage, u0, u_out, dt, rk, boundary_time, nullptr, 0, nullptr, st);
     return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;

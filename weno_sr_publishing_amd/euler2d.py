@@ -110,6 +110,19 @@ class Euler2DSolver:
                                          self.workspace.data_ptr(), self.ws_bytes, self._stream())
         _lib.check(rc, "wsr_euler2d_stage")
 
+    def stage_noalloc(self, u_stage: torch.Tensor, u0: torch.Tensor, u_out: torch.Tensor, dt: float, rk: int,
+                      boundary_time: float = 0.0) -> None:
+        """``stage`` through wsr_euler2d_stage_noalloc: the same kernels on the library's
+        file-scope device scratch instead of this solver's workspace (for callers that
+        cannot allocate; grids up to the scratch size, no closure counting; calls that
+        overlap on one GPU race)."""
+        for t in (u_stage, u0, u_out):
+            self._check_state(t)
+        rc = self._lib.wsr_euler2d_stage_noalloc(ctypes.byref(self.cfg), u_stage.data_ptr(), u0.data_ptr(),
+                                                 u_out.data_ptr(), float(dt), int(rk), float(boundary_time),
+                                                 self._stream())
+        _lib.check(rc, "wsr_euler2d_stage_noalloc")
+
     def max_speed(self, u: torch.Tensor) -> float:
         self._check_state(u)
         out = ctypes.c_double(0.0)
