@@ -389,3 +389,66 @@ def test_oracle_fd_gradient_matches_reference_autograd(oracle, mlp64):
         fm[p] -= h
         fd = (loss(fp) - loss(fm)) / (2 * h)
         assert abs(fd - ref[p]) <= 1e-6 * max(1e-3, np.max(np.abs(ref))), (p, fd, ref[p])
+
+
+# --------------------------------------------------------------------------
+# FORCE (warp_weno5_helpers.py:401-435): the oracle against an independent
+# restatement of Toro's combined form and a closed form that pins the signs
+# and the placement of c = dt/h and 1/c.
+def _euler_flux_conserved(U, direction, gamma=1.4):
+    rho, mx, my, E = U
+    p = (gamma - 1.0) * (E - 0.5 * (mx * mx + my * my) / rho)
+    un = (mx if direction == 1 else my) / rho
+    out = un * np.asarray(U, dtype=np.float64)
+    out[1 if direction == 1 else 2] += p
+    out[3] += p * un
+    return out
+
+
+def _random_state(rng, gamma=1.4):
+    rho = rng.uniform(0.2, 5.0)
+    u, v = rng.normal(scale=1.5, size=2)
+    p = rng.uniform(0.1, 10.0)
+    return np.array([rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)])
+
+
+@pytest.mark.parametrize("direction", [1, 2])
+def test_force_matches_toro_combined_form(oracle, direction):
+    """FORCE = 1/4 [F_L + 2 F(U_RI) + F_R - (h/dt)(U_R - U_L)],
+    U_RI = 1/2 (U_L + U_R) - 1/2 (dt/h) (F_R - F_L)  (Toro, Riemann Solvers, 3rd ed., 7.4.2)."""
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for _ in range(512):
+        ul, ur = _random_state(rng), _random_state(rng)
+        c = rng.uniform(0.02, 0.3)
+        fl, fr = _euler_flux_conserved(ul, direction), _euler_flux_conserved(ur, direction)
+        uri = 0.5 * (ul + ur) - 0.5 * c * (fr - fl)
+        if uri[0] <= 0.0:
+            continue
+        want = 0.25 * (fl + 2.0 * _euler_flux_conserved(uri, direction) + fr - (ur - ul) / c)
+        got = oracle.force(ul, ur, direction, c)
+        scale = max(np.abs(fl).max(), np.abs(fr).max(), np.abs(ur - ul).max() / c)
+        worst = max(worst, float(np.max(np.abs(got - want)) / scale))
+    assert worst <= 1e-13
+
+
+@pytest.mark.parametrize("direction", [1, 2])
+def test_force_on_a_contact_is_the_linear_advection_flux(oracle, direction):
+    """Equal velocity and pressure, different density: along this family the Euler flux
+    is affine, F(U) = un U + (0, p, 0, p un) (p on the normal momentum), and FORCE is the
+    linear closed form 1/2 un (U_L + U_R) + P - 1/4 (c un^2 + 1/c) (U_R - U_L)."""
+    gamma = 1.4
+    rng = np.random.default_rng(5)
+    for _ in range(64):
+        u, v, p = rng.normal(), rng.normal(), rng.uniform(0.5, 3.0)
+        rl, rr = rng.uniform(0.3, 3.0, size=2)
+        c = rng.uniform(0.05, 0.4)
+        mk = lambda rho: np.array([rho, rho * u, rho * v, p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v)])
+        ul, ur = mk(rl), mk(rr)
+        un = u if direction == 1 else v
+        P = np.zeros(4)
+        P[1 if direction == 1 else 2] = p
+        P[3] = p * un
+        want = 0.5 * un * (ul + ur) + P - 0.25 * (c * un * un + 1.0 / c) * (ur - ul)
+        got = oracle.force(ul, ur, direction, c)
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())

@@ -257,3 +257,36 @@ def test_z7_fallback_quiet_on_resolved_smooth_flow(oracle, mlp7):
     oracle.rhs7_2d(u, n, n, 20.0 / n, 20.0 / n, 1.4, 0.1, 0.0, scheme="sr64", boundary="periodic", mlp=mlp7,
                    z_fallback=True)
     assert oracle.fallbacks7() == 0
+
+
+@pytest.mark.parametrize("direction", [1, 2])
+def test_evilin_2d_equals_the_pinned_3d_evilin_without_w(oracle, direction):
+    """The WENO7 2-D EVILIN flux (warp_weno7_ader4_helpers_classical_only.py:256-342) is the
+    3-D EVILIN flux (warp_weno5_3d_rk3/kernels.py:122-219, pinned by the reference's
+    recorded 224 x 88 x 88 run) restricted to states without z-velocity: same four flux
+    components, zero z-momentum flux, for random state pairs in every upwind branch."""
+    rng = np.random.default_rng(3)
+    gamma = 1.4
+    branches = set()
+    for _ in range(400):
+        states = []
+        for _ in range(2):
+            rho = rng.uniform(0.2, 5.0)
+            u, v = rng.normal(scale=3.0, size=2)
+            p = rng.uniform(0.1, 10.0)
+            states.append((rho, u, v, p))
+        c = rng.uniform(0.02, 0.2)
+        u2 = [np.array([r, r * a, r * b, p / (gamma - 1.0) + 0.5 * r * (a * a + b * b)]) for r, a, b, p in states]
+        u3 = [np.array([r, r * a, r * b, 0.0, p / (gamma - 1.0) + 0.5 * r * (a * a + b * b)])
+              for r, a, b, p in states]
+        f2 = oracle.riemann7(u2[0], u2[1], direction, c, gamma, "evilin")
+        f3 = oracle.flux3d(u3[0], u3[1], direction, c, "evilin")
+        want = np.array([f3[0], f3[1], f3[2], f3[4]])
+        assert abs(f3[3]) <= 1e-14 * np.abs(f3).max()
+        assert np.allclose(f2, want, rtol=1e-13, atol=1e-13 * np.abs(want).max()), (states, c)
+        # the upwind region of the averaged pair (EVILIN picks its state by ubar -/+ abar)
+        un = [s[1] if direction == 1 else s[2] for s in states]
+        ubar = 0.5 * (un[0] + un[1])
+        abar = np.sqrt(gamma * 0.5 * (states[0][3] + states[1][3]) / (0.5 * (states[0][0] + states[1][0])))
+        branches.add(0 if ubar - abar >= 0 else (1 if ubar >= 0 else (2 if ubar + abar >= 0 else 3)))
+    assert branches == {0, 1, 2, 3}
