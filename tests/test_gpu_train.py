@@ -105,3 +105,95 @@ def test_training_step_reduces_loss(gpu_device):
     losses = [tr.step(torch.tensor(u0), torch.tensor(vel), torch.tensor(target), dts) for _ in range(8)]
     print("losses", losses)
     assert np.all(np.isfinite(losses)) and losses[-1] < losses[0]
+
+
+def test_rollout_distance_backward_matches_reference_autograd_fp32(gpu_device):
+    """mse(rollout_distance(model, u, 3.7, 0.6, v), target).backward() with the FP32
+    closure module: the parameter gradients of the reference's own autograd
+    (rk3_advection.rollout_distance with model.ReflectionSymmetricBadnessMLPFloat32)."""
+    from weno_sr_publishing_amd.advection1d import rollout_distance
+    from weno_sr_publishing_amd.checkpoints import WENO5_SR_FP32
+    from weno_sr_publishing_amd.model import NAMES, load_checkpoint
+    fx = fixture()
+    model = load_checkpoint(WENO5_SR_FP32, gpu_device)
+    u = torch.tensor(fx["u0"], device=gpu_device)
+    v = torch.tensor(fx["velocities"], device=gpu_device)
+    final = rollout_distance(model, u, float(fx["distance"]), float(fx["cfl"]), v)
+    loss = torch.mean((final - torch.tensor(fx["target"], device=gpu_device)) ** 2)
+    loss.backward()
+    got = np.concatenate([getattr(model, n).grad.double().cpu().numpy().reshape(-1) for n in NAMES])
+    ref = flat_grad(fx, "fp32")
+    rel = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    fin = np.max(np.abs(final.detach().cpu().numpy() - fx["fp32_final"]))
+    print(f"autograd fp32: grad rel {rel:.3e}, final diff {fin:.3e}", flush=True)
+    assert fin <= 1e-6 and rel <= 1e-5
+
+
+def test_chained_steps_backward_matches_reference_autograd_fp64(gpu_device):
+    """Seven chained ssprk3_step_signed calls (one autograd node each) with the FP64
+    closure module and an initial state that requires grad: the reference's
+    v12_losses.ssprk3_step_signed autograd gradients of the parameters and of u0."""
+    from weno_sr_publishing_amd.advection1d import ssprk3_step_signed
+    from weno_sr_publishing_amd.checkpoints import WENO5_SR_FP64
+    from weno_sr_publishing_amd.model import NAMES, load_checkpoint
+    fx = fixture()
+    n = int(fx["n"])
+    model = load_checkpoint(WENO5_SR_FP64, gpu_device)
+    u0 = torch.tensor(fx["u0"], device=gpu_device, requires_grad=True)
+    v = torch.tensor(fx["velocities"], device=gpu_device)
+    state = u0
+    for dt in fx["dts"]:
+        state = ssprk3_step_signed(model, state, float(dt), float(n), v)
+    loss = torch.mean((state - torch.tensor(fx["target"], device=gpu_device)) ** 2)
+    loss.backward()
+    got = np.concatenate([getattr(model, name).grad.cpu().numpy().reshape(-1) for name in NAMES])
+    ref = flat_grad(fx, "fp64")
+    rel = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    ref_u = fx["fp64_grad_u0"]
+    rel_u = np.max(np.abs(u0.grad.cpu().numpy() - ref_u)) / np.max(np.abs(ref_u))
+    print(f"autograd fp64 chained: grad rel {rel:.3e}, u0 grad rel {rel_u:.3e}, loss {loss.item()!r}", flush=True)
+    assert abs(loss.item() - float(fx["fp64_loss"])) <= 1e-14
+    assert rel <= 1e-8 and rel_u <= 1e-9
+
+
+def test_autograd_gradient_at_training_length_matches_oracle_differences(oracle, gpu_device):
+    """The training workload's rollout length (N = 256, 64 cells at CFL 0.6 = 107 steps),
+    a 16-trajectory slice: the autograd gradient of the FP64 closure module along three
+    random parameter directions against central differences of the oracle's FP64
+    forward rollout."""
+    from weno_sr_publishing_amd.advection1d import cfl_schedule, rollout_distance
+    from weno_sr_publishing_amd.checkpoints import WENO5_SR_FP64, MlpParams
+    from weno_sr_publishing_amd.model import NAMES, SHAPES, load_checkpoint
+    from weno_sr_publishing_amd.profiles import make_batch, shifted_target
+    n, batch, distance = 256, 16, 64.0
+    u0, vel = make_batch(batch, n, seed=83)
+    target = shifted_target(u0, vel, 64)
+    dts = [c / n for c in cfl_schedule(distance, 0.6)]
+    assert len(dts) == 107
+    model = load_checkpoint(WENO5_SR_FP64, gpu_device)
+    final = rollout_distance(model, torch.tensor(u0, device=gpu_device), distance, 0.6,
+                             torch.tensor(vel, device=gpu_device))
+    loss = torch.mean((final - torch.tensor(target, device=gpu_device)) ** 2)
+    loss.backward()
+    g = np.concatenate([getattr(model, name).grad.cpu().numpy().reshape(-1) for name in NAMES])
+    flat = np.concatenate([getattr(model, name).detach().cpu().numpy().reshape(-1) for name in NAMES])
+
+    def loss_of(f):
+        arrays, o = {}, 0
+        for name in NAMES:
+            size = int(np.prod(SHAPES[name]))
+            arrays[name] = f[o:o + size].reshape((1,) + SHAPES[name])
+            o += size
+        fin = oracle.adv1d_run(u0, vel, dts, n, "sr64", MlpParams(arrays, "fp64"))
+        return float(np.mean((fin - target) ** 2))
+
+    assert abs(loss_of(flat) - float(loss)) <= 1e-13 * float(loss)
+    rng = np.random.default_rng(29)
+    for _ in range(3):
+        d = rng.normal(size=189)
+        d /= np.linalg.norm(d)
+        h = 1e-6
+        fd = (loss_of(flat + h * d) - loss_of(flat - h * d)) / (2 * h)
+        ad = float(g @ d)
+        print(f"directional derivative: autograd {ad:.10e}, central difference {fd:.10e}", flush=True)
+        assert abs(ad - fd) <= 1e-5 * np.linalg.norm(g)

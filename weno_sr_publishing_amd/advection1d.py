@@ -5,7 +5,17 @@ Entry points mirror the reference's torch rollout
 ``balanced_velocities`` (:27-33), ``ssprk3_step_signed`` (:118-131),
 ``cfl_schedule`` (:149-168) and ``rollout_distance`` (:171-186); ``integrate_ssprk3``
 and ``run_gste`` follow code/snapshot/for_paper_results/run_gste.py:103-118,179-264.
-A "model" argument is ``None``/"classical" (WENO5-JS) or an ``MlpParams``.
+
+A "model" argument is ``None``/"classical" (WENO5-JS), an ``MlpParams``, or a torch
+module carrying the closure parameters ``w1 .. b4`` (``model.load_checkpoint``, or the
+reference's own closure modules).  With a module, ``ssprk3_step_signed`` and
+``rollout_distance`` are differentiable: the rollout is one ``torch.autograd.Function``
+whose forward is the HIP rollout kernel (every step's state checkpointed) and whose
+backward is the hand-written adjoint kernel (each step recomputed from its checkpoint,
+then reversed through the stages and the MLP), giving the gradients of the initial
+state and of the module's parameters, as the reference's loss.backward() through its
+checkpointed torch rollout does (fast_losses.py:412-648).  Classical and
+``MlpParams`` models are not differentiable here: a state that requires grad raises.
 """
 from __future__ import annotations
 
@@ -76,6 +86,82 @@ class Adv1DSolver:
         return ck
 
 
+CLOSURE_NAMES = ("w1", "b1", "w2", "b2", "w3", "b3", "w4", "b4")
+
+
+def _closure_module(model) -> bool:
+    return isinstance(model, torch.nn.Module) and all(hasattr(model, n) for n in CLOSURE_NAMES)
+
+
+class _Rollout1D(torch.autograd.Function):
+    """u_T = rollout(u_0; theta) over the step schedule ``dts`` (SSPRK3, signed speed
+    per trajectory); backward: the adjoint kernel with lambda_T = dL/du_T."""
+
+    @staticmethod
+    def forward(ctx, u0, flat, velocities, dts, scheme, dxinv):
+        L = _lib.lib()
+        batch, n = u0.shape
+        u = u0.detach().to(torch.float64).contiguous().clone()
+        p = flat.detach().contiguous()
+        ck = torch.empty((batch, dts.numel() + 1, n), dtype=torch.float64, device=u.device)
+        cfg = _lib.Adv1DConfig(batch=batch, n=n, scheme=_lib.SCHEMES[scheme], reserved=0, dxinv=float(dxinv),
+                               mlp=None)
+        stream = torch.cuda.current_stream(u.device).cuda_stream
+        rc = L.wsr_adv1d_train_forward(ctypes.byref(cfg), u.data_ptr(), velocities.data_ptr(), dts.data_ptr(),
+                                       dts.numel(), ck.data_ptr(), p.data_ptr(), stream)
+        _lib.check(rc, "wsr_adv1d_train_forward")
+        ctx.save_for_backward(ck, p, velocities, dts)
+        ctx.cfg = cfg
+        return u
+
+    @staticmethod
+    def backward(ctx, grad_u):
+        ck, p, velocities, dts = ctx.saved_tensors
+        cfg = ctx.cfg
+        batch, n = cfg.batch, cfg.n
+        lam = grad_u.to(torch.float64).contiguous()
+        grad_b = torch.empty((batch, 189), dtype=torch.float64, device=lam.device)
+        lam0 = torch.empty((batch, n), dtype=torch.float64, device=lam.device) if ctx.needs_input_grad[0] else None
+        stream = torch.cuda.current_stream(lam.device).cuda_stream
+        rc = _lib.lib().wsr_adv1d_train_adjoint(ctypes.byref(cfg), ck.data_ptr(), lam.data_ptr(),
+                                                velocities.data_ptr(), dts.data_ptr(), dts.numel(), p.data_ptr(),
+                                                grad_b.data_ptr(), lam0.data_ptr() if lam0 is not None else None,
+                                                stream)
+        _lib.check(rc, "wsr_adv1d_train_adjoint")
+        grad_p = grad_b.sum(dim=0).to(p.dtype) if ctx.needs_input_grad[1] else None
+        return lam0, grad_p, None, None, None, None
+
+
+def _rollout(model, state: torch.Tensor, dts, dxinv: float, velocities) -> torch.Tensor:
+    """The rollout for any model; differentiable for a closure module."""
+    if state.dim() != 2:
+        raise ValueError("state must be a (batch, n) tensor")
+    if state.device.type != "cuda":
+        raise ValueError("the 1-D rollout runs on the GPU: pass the state on a cuda device")
+    batch, n = state.shape
+    vel = torch.as_tensor(velocities, dtype=torch.float64, device=state.device).reshape(-1)
+    if vel.numel() == 1:
+        vel = vel.expand(batch)
+    vel = vel.contiguous()
+    if vel.numel() != batch:
+        raise ValueError("one velocity per trajectory")
+    if _closure_module(model):
+        flat = torch.cat([getattr(model, name).reshape(-1) for name in CLOSURE_NAMES])
+        if flat.numel() != 189:
+            raise ValueError("the closure module must be the 5-10-6-6-3 MLP (189 parameters)")
+        if flat.dtype not in (torch.float32, torch.float64):
+            raise ValueError("the closure parameters must be float32 or float64")
+        scheme = "sr32" if flat.dtype == torch.float32 else "sr64"
+        dts_t = torch.as_tensor(np.asarray(dts, dtype=np.float64), device=state.device).reshape(-1).contiguous()
+        return _Rollout1D.apply(state, flat.to(state.device), vel, dts_t, scheme, float(dxinv))
+    if state.requires_grad or (isinstance(model, torch.nn.Module) and any(p.requires_grad for p in model.parameters())):
+        raise ValueError("gradients through the 1-D rollout need a closure module with parameters w1 .. b4 "
+                         "(model.load_checkpoint); classical and MlpParams models are not differentiable here")
+    out = state.detach().to(torch.float64).clone().contiguous()
+    Adv1DSolver(n, dxinv, model, state.device).rollout(out, vel, dts)
+    return out
+
+
 def balanced_velocities(batch: int, device) -> torch.Tensor:
     if batch < 2 or batch % 2 != 0:
         raise ValueError("bidirectional batches must be positive and even")
@@ -85,9 +171,8 @@ def balanced_velocities(batch: int, device) -> torch.Tensor:
 
 
 def ssprk3_step_signed(model, state: torch.Tensor, dt: float, dxinv: float, velocities) -> torch.Tensor:
-    out = state.detach().clone().contiguous()
-    Adv1DSolver(state.shape[1], dxinv, model, state.device).rollout(out, velocities, [dt])
-    return out
+    """One signed SSPRK3 step (rk3_advection.py:118-131); differentiable for a closure module."""
+    return _rollout(model, state, [dt], dxinv, velocities)
 
 
 def cfl_schedule(distance_cells: float, cfl: float) -> tuple[float, ...]:
@@ -113,12 +198,12 @@ def cfl_schedule(distance_cells: float, cfl: float) -> tuple[float, ...]:
 
 
 def rollout_distance(model, initial: torch.Tensor, distance_cells: float, cfl: float, velocities) -> torch.Tensor:
+    """Exactly ``distance_cells`` cell widths with a shortened last step (rk3_advection.py:171-186),
+    as one differentiable rollout for a closure module."""
     n = initial.shape[1]
     dx = 1.0 / float(n)
-    out = initial.detach().clone().contiguous()
     dts = [c * dx for c in cfl_schedule(distance_cells, cfl)]
-    Adv1DSolver(n, 1.0 / dx, model, initial.device).rollout(out, velocities, dts)
-    return out
+    return _rollout(model, initial, dts, 1.0 / dx, velocities)
 
 
 def integrate_ssprk3(model, u0: np.ndarray, cfl_limit: float, t_end: float, device="cuda",
