@@ -58,3 +58,86 @@ def test_shockbubble3d_run_solver_writes_step(tmp_path, gpu_device):
     assert (tmp_path / "dt_trace.csv").exists() and (tmp_path / "run_manifest.json").exists()
     rep2 = run_solver(cfg, device="cuda:0", out_dir=tmp_path / "again", reference=rep["output"])
     assert rep2["comparison"]["array_bitwise_identical"]
+
+
+@pytest.mark.parametrize("module,closure,boundary", [("weno5_hllc", None, "transmissive"),
+                                                     ("weno5_hllc", "fp64", "periodic"),
+                                                     ("weno5_hllc_refsym", "fp64", "double-mach"),
+                                                     ("weno5_hllc_mixed", "fp32", "double-mach")])
+def test_stage_api_matches_oracle(oracle, mlp64, mlp32, gpu_device, module, closure, boundary):
+    """The reference's stage-level API (weno5_hllc.py:361-386, weno5_hllc_mixed.py:121-182,
+    weno5_hllc_refsym.py:298-526): allocate_forward -> launch_stage x 3 (rk 1, 2, 3 with
+    the boundary times t, t + dt, t + dt/2) -> advance_step, dt from compute_dt, against
+    the oracle's SSPRK3 step; then the stage cache follows the parameter values."""
+    import importlib
+    import torch
+    from weno_sr_publishing_amd.initial import make_double_mach_state, make_quadrant_initial
+    from weno_sr_publishing_amd.params import Params
+    mod = importlib.import_module(f"weno_sr_publishing_amd.solvers.{module}")
+    mlp = {None: None, "fp64": mlp64, "fp32": mlp32}[closure]
+    scheme = {None: "js", "fp64": "sr64", "fp32": "sr32"}[closure]
+    if boundary == "double-mach":
+        nx, ny = 64, 16
+        p = Params(nx=nx, ny=ny, x_length=4.0, y_length=1.0, cfl=0.4, t_end=0.2)
+        u0 = make_double_mach_state(nx, ny, ghost=3)
+    else:
+        nx, ny = 48, 40
+        p = Params(nx=nx, ny=ny, x_length=1.0, y_length=1.0, cfl=0.4, t_end=0.25)
+        u0 = make_quadrant_initial(nx, ny, "c4")
+    tol = 1e-12 if scheme != "sr32" else 2e-6
+    arrays = mod.allocate_forward(u0, p, "cuda")
+    assert arrays["speed"].shape == (nx * ny,) and arrays["u0"].shape == p.padded_shape
+    ref, t = u0.copy(), 0.0
+    for step in range(3):
+        dt = mod.compute_dt(arrays["u0"], p, "cuda")
+        # the FP32 closure's state differs from the oracle's within its bar, so its dt does too
+        assert abs(dt - oracle.dt2d(ref, nx, ny, p.dx, p.dy, p.gamma, p.cfl)) <= (0.0 if scheme != "sr32" else 1e-6 * dt)
+        if step == 0:
+            times = (t, t + dt, t + 0.5 * dt)
+            for rk, (src, dst, stage) in enumerate((("u0", "u1", "s1"), ("u1", "u2", "s2"), ("u2", "u3", "s3")), 1):
+                if module == "weno5_hllc":
+                    mod.launch_stage(arrays[src], arrays["u0"], arrays[dst], arrays[stage], p, dt, rk, "cuda", mlp,
+                                     boundary)
+                elif module == "weno5_hllc_refsym":
+                    mod.launch_stage(arrays[src], arrays["u0"], arrays[dst], arrays[stage], p, dt, rk, "cuda", mlp,
+                                     boundary, times[rk - 1], False)
+                else:
+                    mod.launch_stage(arrays[src], arrays["u0"], arrays[dst], arrays[stage], p, dt, rk, "cuda", mlp,
+                                     boundary, times[rk - 1])
+            arrays["u0"], arrays["u3"] = arrays["u3"], arrays["u0"]
+        elif module == "weno5_hllc":
+            mod.advance_step(arrays, p, dt, "cuda", mlp, boundary)
+        else:
+            mod.advance_step(arrays, p, dt, "cuda", mlp, boundary, t)
+        ref = oracle.step2d(ref, nx, ny, p.dx, p.dy, p.gamma, dt, t, scheme=scheme, boundary=boundary, mlp=mlp)
+        t += dt
+        got = arrays["u0"].cpu().numpy()
+        err = float(np.max(np.abs(got[3:-3, 3:-3] - ref[3:-3, 3:-3])) / np.max(np.abs(ref[3:-3, 3:-3])))
+        print(f"{module} {scheme} {boundary} step {step + 1}: {err:.3e}", flush=True)
+        assert err <= tol
+    if mlp is not None:
+        # the same stage dict with other weights (a new record, and the same record
+        # changed in place) must not reuse the solver built for the old ones
+        from weno_sr_publishing_amd.checkpoints import MlpParams
+        changed = MlpParams({k: np.array(v) * (1.0 + 1e-3) for k, v in mlp.items()}, mlp.precision)
+        inplace = MlpParams({k: np.array(v) for k, v in mlp.items()}, mlp.precision)
+        state = arrays["u0"].clone()
+        outs = []
+        for m in (mlp, changed, inplace):
+            if m is inplace:
+                for v in inplace.values():
+                    v *= 1.0 + 1e-3
+            out = torch.empty_like(state)
+            if module == "weno5_hllc":
+                mod.launch_stage(state, state, out, arrays["s1"], p, 1e-4, 1, "cuda", m, boundary)
+            elif module == "weno5_hllc_refsym":
+                mod.launch_stage(state, state, out, arrays["s1"], p, 1e-4, 1, "cuda", m, boundary, t, False)
+            else:
+                mod.launch_stage(state, state, out, arrays["s1"], p, 1e-4, 1, "cuda", m, boundary, t)
+            outs.append(out.cpu().numpy())
+        want = oracle.stage2d(state.cpu().numpy(), state.cpu().numpy(), nx, ny, p.dx, p.dy, p.gamma, 1e-4, 1,
+                              scheme=scheme, boundary=boundary, boundary_time=t, mlp=changed)
+        assert not np.array_equal(outs[0][3:-3, 3:-3], outs[1][3:-3, 3:-3])
+        assert np.array_equal(outs[1][3:-3, 3:-3], outs[2][3:-3, 3:-3])   # the stage writes the interior
+        err = float(np.max(np.abs(outs[1][3:-3, 3:-3] - want[3:-3, 3:-3])) / np.max(np.abs(want[3:-3, 3:-3])))
+        assert err <= tol

@@ -227,3 +227,24 @@ def test_full_size_periodic_conservation_and_symmetry(mlp32, gpu_device, scheme)
     sym = np.max(np.abs(a - at)) / np.max(np.abs(a))
     print(f"transpose symmetry {scheme}: {sym:.3e}")
     assert sym <= (1e-11 if scheme == "js" else 1e-4)
+
+
+@pytest.mark.parametrize("scheme", ["sr64", "sr32", "srbf16"])
+def test_full_size_developed_flow_one_step_parity(oracle, mlp64, mlp32, gpu_device, scheme):
+    """400x400 C.4 from the developed flow at t = 0.125 (the GPU's own trajectory for the
+    closure; 31-41 % of stencils run the closure there), one SSPRK3 step on the GPU and in
+    the oracle from the same state: the whole field within the closure's bar."""
+    nx = ny = 400
+    p = Params(nx=nx, ny=ny, x_length=1.0, y_length=1.0, cfl=0.4, t_end=0.25)
+    mlp = {"sr64": mlp64, "sr32": mlp32, "srbf16": mlp32.as_precision("bf16")}[scheme]
+    s = solver(p, scheme, mlp)
+    u = s.new_state(make_quadrant_initial(nx, ny, "c4"))
+    s.run(u, 0.125)
+    start = u.cpu().numpy()
+    dts, steps, t = s.run(u, p.t_end, max_steps=1)
+    ref, rdts, _, _ = oracle.run2d(start, nx, ny, p.dx, p.dy, GAMMA, p.cfl, p.t_end, scheme=scheme, mlp=mlp,
+                                   max_steps=1)
+    assert steps == 1 and abs(dts[0] - rdts[0]) <= 1e-15 * rdts[0]
+    err = rel_err(u.cpu().numpy(), ref)
+    print(f"400^2 developed-flow step parity {scheme}: {err:.3e}", flush=True)
+    assert err <= TOL[scheme]

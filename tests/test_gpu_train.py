@@ -197,3 +197,27 @@ def test_autograd_gradient_at_training_length_matches_oracle_differences(oracle,
         ad = float(g @ d)
         print(f"directional derivative: autograd {ad:.10e}, central difference {fd:.10e}", flush=True)
         assert abs(ad - fd) <= 1e-5 * np.linalg.norm(g)
+
+
+def test_training_resume_is_bit_identical(gpu_device, tmp_path):
+    """Save the resume state after 2 of 4 steps, drop the trainer, resume in a fresh one
+    (built from another seed: everything must come from the file): the losses of steps
+    3 and 4 and the final parameters equal the uninterrupted run bit for bit.  A resume
+    with another recipe signature is refused; the save leaves no temporary file."""
+    from weno_sr_publishing_amd.training import ClosureTrainer, train
+    kw = dict(n=64, batch=16, distance_cells=8, cfl=0.6, seed=83)
+    full = ClosureTrainer(None, "fp32", lr=3e-3, device=gpu_device, seed=41)
+    losses_full = train(full, steps=4, **kw)
+    first = ClosureTrainer(None, "fp32", lr=3e-3, device=gpu_device, seed=41)
+    state = tmp_path / "state.npz"
+    losses_a = train(first, steps=2, state_path=state, **kw)
+    del first
+    second = ClosureTrainer(None, "fp32", lr=3e-3, device=gpu_device, seed=7)
+    losses_b = train(second, steps=4, resume_from=state, **kw)
+    print("losses", losses_full, losses_a + losses_b, flush=True)
+    assert losses_a + losses_b == losses_full
+    assert torch.equal(second.param, full.param)
+    assert sorted(p.name for p in tmp_path.iterdir()) == ["state.npz"]
+    with pytest.raises(ValueError, match="signature mismatch"):
+        train(ClosureTrainer(None, "fp32", lr=3e-3, device=gpu_device), steps=5, resume_from=state,
+              **{**kw, "cfl": 0.5})

@@ -57,3 +57,26 @@ def test_rollout_refuses_host_state_and_non_differentiable_models():
     u = torch.zeros((2, 16), dtype=torch.float64)
     with pytest.raises(ValueError, match="GPU"):
         advection1d.ssprk3_step_signed(None, u, 0.01, 16.0, torch.tensor([1.0, -1.0]))
+
+
+def test_resume_file_is_plain_arrays_and_refuses_other_recipes(tmp_path):
+    """The resume file loads without unpickling; a signature or recipe mismatch raises
+    before any state is touched (no GPU call: the trainer's parameters live on the CPU
+    here)."""
+    import json
+    from weno_sr_publishing_amd.training import ClosureTrainer, training_signature
+    tr = ClosureTrainer(None, "fp64", lr=1e-3, device="cpu")
+    sig = training_signature(64, 16, 8, 0.6, "fp64", 1e-3, 83)
+    gen = torch.Generator().manual_seed(83)
+    tr.save_state(tmp_path / "s.npz", 5, sig, gen)
+    with np.load(tmp_path / "s.npz", allow_pickle=False) as data:
+        meta = json.loads(str(data["meta_json"]))
+        assert meta["step"] == 5 and meta["training_signature"] == sig
+        assert data["param"].shape == (189,) and data["generator_state"].dtype == np.uint8
+    other = ClosureTrainer(None, "fp64", lr=1e-3, device="cpu", seed=3)
+    before = other.param.clone()
+    with pytest.raises(ValueError, match="signature mismatch"):
+        other.resume(tmp_path / "s.npz", {**sig, "batch": 32})
+    assert torch.equal(other.param, before)
+    assert other.resume(tmp_path / "s.npz", sig, torch.Generator()) == 5
+    assert torch.equal(other.param, tr.param)

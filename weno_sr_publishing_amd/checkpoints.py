@@ -31,8 +31,9 @@ class MlpParams(dict):
     """``{"w1": ..., "b4": ...}`` plus the packed parameter block the kernels take.
 
     ``packed`` is a contiguous host array of the 189 parameters in the order
-    w1 (row-major k,o), b1, w2, b2, w3, b3, w4, b4 and in the closure dtype.
-    ``precision`` is "fp64", "fp32" or "bf16".
+    w1 (row-major k,o), b1, w2, b2, w3, b3, w4, b4 and in the closure dtype, built
+    from the current arrays each time it is read (weights changed in place reach the
+    next solver that packs them).  ``precision`` is "fp64", "fp32" or "bf16".
     """
 
     def __init__(self, arrays: dict, precision: str, meta: dict | None = None):
@@ -41,11 +42,14 @@ class MlpParams(dict):
             raise ValueError(f"unknown closure precision {precision!r}")
         self.precision = precision
         self.meta = meta or {}
-        dtype = np.float64 if precision == "fp64" else np.float32
-        flat = np.concatenate([np.asarray(arrays[n]).reshape(-1) for n in REQUIRED]).astype(dtype)
-        if flat.size != MLP5_COUNT:
-            raise ValueError(f"WENO5 MLP must have {MLP5_COUNT} parameters, got {flat.size}")
-        self.packed = np.ascontiguousarray(flat)
+        if self.packed.size != MLP5_COUNT:
+            raise ValueError(f"WENO5 MLP must have {MLP5_COUNT} parameters, got {self.packed.size}")
+
+    @property
+    def packed(self) -> np.ndarray:
+        dtype = np.float64 if self.precision == "fp64" else np.float32
+        flat = np.concatenate([np.asarray(self[n]).reshape(-1) for n in REQUIRED]).astype(dtype)
+        return np.ascontiguousarray(flat)
 
     def as_precision(self, precision: str) -> "MlpParams":
         """Same parameters evaluated at another closure precision (fp64 -> fp32 casts etc.)."""

@@ -29,10 +29,12 @@ def launch_stage(u_stage, u0, u_out, stage: dict, params: Params, dt: float, rk:
 
 
 def advance_step(arrays: dict, params: Params, dt: float, device, mlp_params, boundary: str) -> None:
-    launch_stage(arrays["u0"], arrays["u0"], arrays["u1"], arrays["s1"], params, dt, 1, device, mlp_params, boundary)
-    launch_stage(arrays["u1"], arrays["u0"], arrays["u2"], arrays["s2"], params, dt, 2, device, mlp_params, boundary)
-    launch_stage(arrays["u2"], arrays["u0"], arrays["u3"], arrays["s3"], params, dt, 3, device, mlp_params, boundary)
-    arrays["u0"], arrays["u3"] = arrays["u3"], arrays["u0"]
+    """weno5_hllc.py:375-386: three stage launches, then u0 <- u3."""
+    _check_boundary(boundary)
+    _adapter.advance_step(arrays, params, dt, device, mlp_params, boundary, scheme_for(mlp_params))
+
+
+compute_dt = _adapter.compute_dt
 
 
 def run_to_time(u0, params: Params, t_end: float, device, mlp_params, boundary: str,

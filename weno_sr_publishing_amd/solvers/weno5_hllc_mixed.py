@@ -26,6 +26,15 @@ def launch_stage(u_stage, u0, u_out, stage: dict, params: Params, dt: float, rk:
                           _scheme(mlp_params), boundary_time, False)
 
 
+def advance_step(arrays: dict, params: Params, dt: float, device, mlp_params, boundary: str,
+                 time: float = 0.0) -> None:
+    """Three stage launches (boundary times t, t + dt, t + dt/2), then u0 <- u3."""
+    _adapter.advance_step(arrays, params, dt, device, mlp_params, boundary, _scheme(mlp_params), time)
+
+
+compute_dt = _adapter.compute_dt
+
+
 def run_to_time(u0, params: Params, t_end: float, device, mlp_params, boundary: str,
                 report_interval: int = 0, report: Callable | None = None):
     return _adapter.run_to_time(u0, params, t_end, device, mlp_params, boundary, _scheme(mlp_params),

@@ -23,6 +23,17 @@ def launch_stage(u_stage, u0, u_out, stage: dict, params: Params, dt: float, rk:
                           scheme_for(mlp_params), boundary_time, eno_cutoff)
 
 
+def advance_step(arrays: dict, params: Params, dt: float, device, mlp_params, boundary: str,
+                 time: float = 0.0, eno_cutoff: bool = False) -> None:
+    """weno5_hllc_refsym.py:486-526: three stage launches with the boundary times
+    t, t + dt, t + dt/2, then u0 <- u3."""
+    _adapter.advance_step(arrays, params, dt, device, mlp_params, boundary, scheme_for(mlp_params), time,
+                          eno_cutoff)
+
+
+compute_dt = _adapter.compute_dt
+
+
 def run_to_time(initial, params: Params, t_end: float, device, mlp_params, boundary: str,
                 eno_cutoff: bool = False, report_interval: int = 100,
                 report: Callable | None = None):

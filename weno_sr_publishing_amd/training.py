@@ -8,13 +8,24 @@ autograd through checkpointed chunks of the rollout
 train.py:449-575).
 
 Loss of one batch: mean over trajectories and cells of (u_T - target)^2 where u_T
-is the state after the whole step schedule.
+is the state after the whole step schedule.  Other losses go through the
+differentiable drop-ins (``advection1d.rollout_distance`` with a closure module and
+``loss.backward()``), which use the same kernels.
+
+Resume state (train.py:47-58, 202-229, 290-318 of the reference): ``save_state``
+writes the parameters, the Adam moments and step count, the training step, the batch
+generator's state and a recipe signature to one ``.npz`` (arrays plus a JSON record;
+no pickled objects) through a temporary file and ``os.replace``; ``resume`` refuses a
+file whose recipe or signature differs from the caller's.
 """
 from __future__ import annotations
 
 import ctypes
 import json
 import math
+import os
+import tempfile
+from pathlib import Path
 
 import numpy as np
 import torch
@@ -131,6 +142,66 @@ class ClosureTrainer:
         self.opt.step()
         return float(loss)
 
+    # ------------------------------------------------------------ resume state
+    def save_state(self, path, step: int, signature: dict, generator: torch.Generator | None = None) -> None:
+        """Atomic resume file: parameters, Adam state, step, generator state, signature."""
+        path = Path(path)
+        path.parent.mkdir(parents=True, exist_ok=True)
+        st = self.opt.state.get(self.param, {})
+        group = self.opt.param_groups[0]
+        meta = {"format_version": 1, "recipe": RECIPE, "step": int(step), "precision": self.precision,
+                "training_signature": signature, "adam": {"lr": group["lr"], "betas": list(group["betas"]),
+                                                          "eps": group["eps"], "weight_decay": group["weight_decay"]},
+                "adam_step": float(st["step"]) if "step" in st else 0.0}
+        arrays = {"param": self.param.detach().cpu().numpy(),
+                  "meta_json": np.array(json.dumps(meta, sort_keys=True), dtype=np.str_)}
+        if "exp_avg" in st:
+            arrays["exp_avg"] = st["exp_avg"].detach().cpu().numpy()
+            arrays["exp_avg_sq"] = st["exp_avg_sq"].detach().cpu().numpy()
+        if generator is not None:
+            arrays["generator_state"] = generator.get_state().numpy()
+        handle, tmp = tempfile.mkstemp(prefix=path.name + ".", suffix=".tmp", dir=path.parent)
+        os.close(handle)
+        try:
+            with open(tmp, "wb") as f:
+                np.savez(f, **arrays)
+            os.replace(tmp, path)
+        finally:
+            if os.path.exists(tmp):
+                os.unlink(tmp)
+
+    def resume(self, path, signature: dict, generator: torch.Generator | None = None) -> int:
+        """Restore a save_state file; returns the step it was saved at."""
+        with np.load(path, allow_pickle=False) as data:
+            meta = json.loads(str(data["meta_json"]))
+            if meta.get("recipe") != RECIPE:
+                raise ValueError("incompatible resume recipe")
+            if meta.get("training_signature") != json.loads(json.dumps(signature, sort_keys=True)):
+                raise ValueError("resume training signature mismatch")
+            if meta.get("precision") != self.precision:
+                raise ValueError("resume precision mismatch")
+            param = np.array(data["param"])
+            moments = (np.array(data["exp_avg"]), np.array(data["exp_avg_sq"])) if "exp_avg" in data.files else None
+            gen_state = np.array(data["generator_state"]) if "generator_state" in data.files else None
+        with torch.no_grad():
+            self.param.copy_(torch.as_tensor(param, device=self.device))
+        group = self.opt.param_groups[0]
+        group["lr"] = meta["adam"]["lr"]
+        group["betas"] = tuple(meta["adam"]["betas"])
+        group["eps"] = meta["adam"]["eps"]
+        group["weight_decay"] = meta["adam"]["weight_decay"]
+        self.opt.state.clear()
+        if moments is not None:
+            self.opt.state[self.param] = {
+                "step": torch.tensor(meta["adam_step"], dtype=torch.float32),
+                "exp_avg": torch.as_tensor(moments[0], device=self.device),
+                "exp_avg_sq": torch.as_tensor(moments[1], device=self.device)}
+        if generator is not None:
+            if gen_state is None:
+                raise ValueError("the resume file holds no generator state")
+            generator.set_state(torch.as_tensor(gen_state, dtype=torch.uint8))
+        return int(meta["step"])
+
     def params(self) -> dict:
         return unflatten(self.param.detach().cpu().numpy())
 
@@ -144,6 +215,46 @@ class ClosureTrainer:
         full.update(meta or {})
         save_mlp_params(path, {k: np.asarray(v, dtype=np.float32 if self.precision == "fp32" else np.float64)
                                for k, v in self.params().items()}, full)
+
+
+RECIPE = "weno5_sr_closure_distance_rollout_mse_hand_written_adjoint"
+
+
+def training_signature(n: int, batch: int, distance_cells: float, cfl: float, precision: str, lr: float,
+                       seed: int) -> dict:
+    """The recipe values a resumed run must share with the run that saved the state."""
+    return {"n": int(n), "batch": int(batch), "distance_cells": float(distance_cells), "cfl": float(cfl),
+            "precision": precision, "lr": float(lr), "seed": int(seed)}
+
+
+def train(trainer: ClosureTrainer, n: int, batch: int, distance_cells: float, cfl: float, steps: int,
+          seed: int = 83, state_path=None, save_every: int = 0, resume_from=None) -> list[float]:
+    """Training loop on solver-generated batches: each step draws a batch seed from a
+    torch generator seeded with ``seed`` (random profiles, a = +1 / -1 balanced,
+    target = the exact shifted averages after ``distance_cells`` cells), then one Adam
+    step.  With ``state_path`` the resume state is written every ``save_every`` steps
+    and at the end; ``resume_from`` continues a saved run (same signature required).
+    Returns the losses of the steps run here."""
+    from .profiles import make_batch, shifted_target
+    if float(distance_cells) != int(distance_cells):
+        raise ValueError("the exact shifted target needs a whole number of cells")
+    sig = training_signature(n, batch, distance_cells, cfl, trainer.precision,
+                             trainer.opt.param_groups[0]["lr"], seed)
+    gen = torch.Generator(device="cpu")
+    gen.manual_seed(seed)
+    start = 0
+    if resume_from is not None:
+        start = trainer.resume(resume_from, sig, gen)
+    dts = distance_schedule(n, distance_cells, cfl)
+    losses = []
+    for step in range(start + 1, steps + 1):
+        batch_seed = int(torch.randint(0, 2 ** 31 - 1, (1,), generator=gen))
+        u0, vel = make_batch(batch, n, batch_seed)
+        target = shifted_target(u0, vel, int(distance_cells))
+        losses.append(trainer.step(torch.tensor(u0), torch.tensor(vel), torch.tensor(target), dts))
+        if state_path is not None and ((save_every and step % save_every == 0) or step == steps):
+            trainer.save_state(state_path, step, sig, gen)
+    return losses
 
 
 def distance_schedule(n: int, distance_cells: float, cfl: float) -> list[float]:
