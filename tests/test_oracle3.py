@@ -189,3 +189,40 @@ def test_3d_transverse_order_is_not_symmetric(oracle):
     for axes, perm in (XY, XZ):
         err = _exchange(oracle, q, n, axes, perm, "hllc")
         assert 1e-8 <= err <= 1e-4, err
+
+
+def test_shockbubble_bench_state():
+    """The 3-D benchmark's initial state (a planar Mach-3 shock into rho = 1, p = 1,
+    striking a light bubble, density ratio 0.138, radius ny/4, in pressure equilibrium):
+    the shocked cells satisfy the Rankine-Hugoniot conditions with the quiescent gas at
+    the shock speed 3 sqrt(1.4) (mass, momentum and energy fluxes, restated here), the
+    unshocked gas has p = 1 inside and outside the bubble, and the bubble's mass deficit
+    is (1 - 0.138) (4/3) pi r^3 within the 3x3x3 quadrature's resolution of the sphere."""
+    from weno_sr_publishing_amd.initial import make_shockbubble_bench_state
+    gamma, mach = 1.4, 3.0
+    nx, ny, nz = 112, 44, 44
+    q = make_shockbubble_bench_state(nx, ny, nz)[3:-3, 3:-3, 3:-3]
+    assert q.shape == (nz, ny, nx, 5)
+    rho = q[..., 0]
+    vel = q[..., 1:4] / rho[..., None]
+    p = (gamma - 1.0) * (q[..., 4] - 0.5 * rho * np.sum(vel * vel, axis=-1))
+    post = q[:, :, :3]                                   # cells entirely behind the shock (x < 3)
+    assert np.ptp(post, axis=(0, 1, 2)).max() == 0.0
+    r2, m2, e2 = post[0, 0, 0, 0], post[0, 0, 0, 1], post[0, 0, 0, 4]
+    u2 = m2 / r2
+    p2 = (gamma - 1.0) * (e2 - 0.5 * r2 * u2 * u2)
+    s = mach * np.sqrt(gamma * 1.0 / 1.0)
+    r1, u1, p1 = 1.0, 0.0, 1.0
+    e1 = p1 / (gamma - 1.0)
+    assert abs(r2 * (u2 - s) - r1 * (u1 - s)) <= 1e-13 * r2 * s
+    assert abs(r2 * u2 * (u2 - s) + p2 - (r1 * u1 * (u1 - s) + p1)) <= 1e-13 * p2
+    assert abs(e2 * (u2 - s) + p2 * u2 - (e1 * (u1 - s) + p1 * u1)) <= 1e-13 * e2 * s
+    ahead = q[:, :, 6:]
+    assert np.all(ahead[..., 1:4] == 0.0)
+    assert np.max(np.abs(p[:, :, 6:] - 1.0)) <= 1e-14
+    assert np.all(vel[..., 1:] == 0.0)
+    deficit = float(np.sum(1.0 - ahead[..., 0]))
+    radius = 0.25 * ny
+    assert abs(deficit / ((1.0 - 0.138) * 4.0 / 3.0 * np.pi * radius ** 3) - 1.0) <= 2e-3
+    # quadrature weights sum to 8 up to rounding: the extremes are 0.138 and 1 to 1e-15
+    assert abs(ahead[..., 0].min() - 0.138) <= 1e-15 and abs(ahead[..., 0].max() - 1.0) <= 1e-15
