@@ -547,6 +547,7 @@ static void enqueue_params(const wsr_euler2d_config *cfg, const Workspace &w, hi
         if (scheme == SCHEME_SRBF16)
             for (int i = 0; i < MLP5_COUNT; ++i) ref[i] = host_round_bf16(ref[i]);
         pack_neuron_major(ref, dev);
+        if (scheme != SCHEME_SRBF16) fold_fp32_closure(dev);   // the FP32 closure evaluates folded parameters
         memcpy(pw.w, dev, sizeof dev);
         nwords = MLP5_COUNT;
     }

@@ -748,6 +748,7 @@ static void enqueue_params3(const wsr_euler3_config *cfg, const Workspace3 &w, h
         if (scheme == WSR_SCHEME_SRBF16)
             for (int i = 0; i < MLP5_COUNT; ++i) ref[i] = host_round_bf16(ref[i]);
         pack_neuron_major(ref, dev);
+        if (scheme != WSR_SCHEME_SRBF16) fold_fp32_closure(dev);   // the FP32 closure evaluates folded parameters
         memcpy(pw.w, dev, sizeof dev);
         nwords = MLP5_COUNT;
     }

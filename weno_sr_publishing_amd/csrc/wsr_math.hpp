@@ -58,6 +58,26 @@ __host__ __device__ inline void pack_neuron_major(const T *ref, T *dev) {
         }
 }
 
+// FP32 closure with its activation constants folded into the packed parameters
+// (host side, at upload; BF16 keeps the plain parameters).  With L = log2(e) the
+// hidden layers compute z' = -L z (layer 1 weights and biases and the biases of
+// layers 2 and 3 scaled by -L), so that the swish is h' = z' / (1 + 2^z') = -L swish(z)
+// with no multiply, and the output layer computes u = (L / 3) z4 (weights scaled by
+// -1/3, which undoes the -L of h', biases by L / 3), the exponent of e^(z4 / 3) that
+// the bounded head 6 tanh(z4 / 6) = 6 - 12 / (1 + e^(z4 / 3)) needs.
+constexpr double LOG2E = 1.4426950408889634;
+inline void fold_fp32_closure(float *dev) {
+    const int start[4] = {DEV_L1, DEV_L2, DEV_L3, DEV_L4}, K[4] = {5, 10, 6, 6}, N[4] = {10, 6, 6, 3};
+    for (int l = 0; l < 4; ++l)
+        for (int n = 0; n < N[l]; ++n) {
+            float *run = dev + start[l] + n * (K[l] + 1);   // [bias, w_0 .. w_(K-1)]
+            const double wscale = l == 0 ? -LOG2E : (l == 3 ? -1.0 / 3.0 : 1.0);
+            const double bscale = l == 3 ? LOG2E / 3.0 : -LOG2E;
+            run[0] = (float)((double)run[0] * bscale);
+            for (int k = 1; k <= K[l]; ++k) run[k] = (float)((double)run[k] * wscale);
+        }
+}
+
 // Parameters are read through the constant address space (scalar loads into
 // SGPRs, one SGPR operand per FMA).  The empty asm makes the pointer opaque
 // before every layer so that the loads stay next to their use instead of being
@@ -411,6 +431,36 @@ __device__ __forceinline__ void softmax_head32(const float z[3], float r[3]) {
     r[0] = e0 * inv; r[1] = e1 * inv; r[2] = e2 * inv;
 }
 
+// The FP32 closure on folded parameters (fold_fp32_closure): swish h' = z' / (1 + 2^z'),
+// head e_k = 2^(6L - 12L / (1 + 2^u_k)) = e^(6 tanh(z_k / 6)) in (e^-6, e^6), softmax
+// without the max shift (the bounded logits cannot overflow).
+__device__ __forceinline__ float swish_folded(float zp) { return zp * rcp_fast(1.0f + __builtin_amdgcn_exp2f(zp)); }
+
+__device__ __forceinline__ void softmax_head_folded(const float u[3], float r[3]) {
+    const float c6 = (float)(6.0 * LOG2E), c12 = (float)(12.0 * LOG2E);
+    float e[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float q = rcp_fast(1.0f + __builtin_amdgcn_exp2f(u[k]));
+        e[k] = __builtin_amdgcn_exp2f(fmaf(-c12, q, c6));
+    }
+    float inv = rcp_fast(e[0] + e[1] + e[2]);
+    r[0] = e[0] * inv; r[1] = e[1] * inv; r[2] = e[2] * inv;
+}
+
+__device__ __forceinline__ void mlp_pair32_folded(const float xa[5], const float xb[5], const float *w, float ra[3],
+                                                  float rb[3]) {
+    float a1[10], b1[10], a2[6], b2[6], a3[6], b3[6], za[3], zb[3];
+    auto sw = [](float z) { return swish_folded(z); };
+    auto id = [](float z) { return z; };
+    dense_pair<5, 10>(xa, xb, w, DEV_L1, a1, b1, sw);
+    dense_pair<10, 6>(a1, b1, w, DEV_L2, a2, b2, sw);
+    dense_pair<6, 6>(a2, b2, w, DEV_L3, a3, b3, sw);
+    dense_pair<6, 3>(a3, b3, w, DEV_L4, za, zb, id);
+    softmax_head_folded(za, ra);
+    softmax_head_folded(zb, rb);
+}
+
 // FP32 (BF16: inputs, weights and activations rounded to bfloat16, FP32 arithmetic)
 template <bool BF16>
 __device__ __forceinline__ void mlp_pair32(const float xa_in[5], const float xb_in[5], const float *w, float ra[3],
@@ -429,6 +479,59 @@ __device__ __forceinline__ void mlp_pair32(const float xa_in[5], const float xb_
     dense_pair<6, 3>(a3, b3, w, DEV_L4, za, zb, id);
     softmax_head32(za, ra);
     softmax_head32(zb, rb);
+}
+
+// SR features for the FP32 closure: the plateau rule in FP64 as sr_features (the branch
+// is exact), then the five features with FP32 reciprocals and one v_log_f32 from the
+// FP64 differences (the reference computes them in FP64 and casts them to FP32;
+// the difference is an FP32 rounding of each feature).
+__device__ __forceinline__ bool sr_features32(const double q[5], float f[5]) {
+    double d20 = q[0] - 2.0 * q[1] + q[2];
+    double d21 = q[1] - 2.0 * q[2] + q[3];
+    double d22 = q[2] - 2.0 * q[3] + q[4];
+    const double c1312 = 13.0 / 12.0;
+    double delta0 = c1312 * fabs(d20) + 0.25 * fabs(q[0] - 4.0 * q[1] + 3.0 * q[2]);
+    double delta1 = c1312 * fabs(d21) + 0.25 * fabs(q[1] - q[3]);
+    double delta2 = c1312 * fabs(d22) + 0.25 * fabs(3.0 * q[2] - 4.0 * q[3] + q[4]);
+    double delta_max = dmax(dmax(delta0, delta1), delta2);
+    double q_scale = dmax(dmax(dmax(fabs(q[0]), fabs(q[1])), dmax(fabs(q[2]), fabs(q[3]))), fabs(q[4]));
+    q_scale = dmax(q_scale, 1.0);
+    if (delta_max <= 1.0e-13 * q_scale) return true;
+    // not a plateau: delta_max > 1e-13 q_scale >= 1e-13, above the reference's 1e-15 floors
+    const double eps = 1.0e-15;
+    float g0 = (float)fabs(d20) * rcp_fast((float)(fabs(q[1] - q[0]) + fabs(q[2] - q[1]) + eps));
+    float g1 = (float)fabs(d21) * rcp_fast((float)(fabs(q[2] - q[1]) + fabs(q[3] - q[2]) + eps));
+    float g2 = (float)fabs(d22) * rcp_fast((float)(fabs(q[3] - q[2]) + fabs(q[4] - q[3]) + eps));
+    float inv_dm = rcp_fast((float)delta_max);
+    float rel = (float)delta_max * rcp_fast((float)q_scale);
+    // (log10(rel) + 4) / 4 = log2(rel) log10(2) / 4 + 1, clipped to [0, 1]
+    float scale = fmaf(__builtin_amdgcn_logf(rel), (float)(0.30102999566398120 / 4.0), 1.0f);
+    f[0] = (float)delta0 * inv_dm;
+    f[1] = (float)delta1 * inv_dm;
+    f[2] = (float)delta2 * inv_dm;
+    f[3] = fminf(1.0f, fmax_(fmax_(g0, g1), g2));
+    f[4] = fminf(1.0f, fmax_(0.0f, scale));
+    return false;
+}
+
+// 1 / x in FP64 from v_rcp_f64 and two Newton steps (the FP32 closure's weights)
+__device__ __forceinline__ double rcp_newton(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// FP32 closure: reflection-averaged ratios from the FP32 features and folded parameters
+template <class MLP>
+__device__ __forceinline__ void sr_ratios32(const float f[5], const MLP &P, double br[3]) {
+    float xr[5] = {f[2], f[1], f[0], f[3], f[4]};
+    float rd[3], rr[3];
+    mlp_pair32_folded(f, xr, P.p, rd, rr);
+    br[0] = 0.5 * ((double)rd[0] + (double)rr[2]);
+    br[1] = 0.5 * ((double)rd[1] + (double)rr[1]);
+    br[2] = 0.5 * ((double)rd[2] + (double)rr[0]);
 }
 
 // Reflection-averaged ratios 0.5 [M(x) + P3 M(P5 x)] in FP64 (C64:225-237, C32:225-237).
@@ -451,14 +554,18 @@ __device__ __forceinline__ void sr_ratios(const double f[5], const MLP &P, doubl
     br[2] = 0.5 * (direct[2] + refl[0]);
 }
 
-// C64:239-279: beta = 3 r, alpha = d / (beta + 1e-12)^2, normalised (+ optional ENO cutoff)
+// C64:239-279: beta = 3 r, alpha = d / (beta + 1e-12)^2, normalised (+ optional ENO cutoff).
+// FAST (the FP32 closure) takes the reciprocals from rcp_newton instead of an IEEE division.
+template <bool FAST = false>
 __device__ __forceinline__ void sr_inv_sq(const double br[3], double ib2[3]) {
     for (int k = 0; k < 3; ++k) {
-        double ib = 1.0 / (3.0 * br[k] + 1.0e-12);
+        double x = 3.0 * br[k] + 1.0e-12;
+        double ib = FAST ? rcp_newton(x) : 1.0 / x;
         ib2[k] = ib * ib;
     }
 }
 
+template <bool FAST = false>
 __device__ __forceinline__ double sr_combine(const double s[3], const double d[3], const double ib2[3],
                                              bool plateau, int eno_cutoff) {
     double w0, w1, w2;
@@ -466,7 +573,8 @@ __device__ __forceinline__ double sr_combine(const double s[3], const double d[3
         w0 = d[0]; w1 = d[1]; w2 = d[2];
     } else {
         double a0 = d[0] * ib2[0], a1 = d[1] * ib2[1], a2 = d[2] * ib2[2];
-        double inv = 1.0 / dmax(a0 + a1 + a2, 1.0e-300);
+        double asum = dmax(a0 + a1 + a2, 1.0e-300);
+        double inv = FAST ? rcp_newton(asum) : 1.0 / asum;
         w0 = a0 * inv; w1 = a1 * inv; w2 = a2 * inv;
         if (eno_cutoff) {
             const double cutoff = 4.0e-7;
@@ -498,6 +606,17 @@ __device__ __forceinline__ double recon_normal(const double q[5], bool lr1, cons
         double tau5 = fabs(b[0] - b[2]), eps_z = h * h;
         for (int k = 0; k < 3; ++k) { double r = tau5 / (b[k] + eps_z); r2[k] = r * r; }
         return z_combine(s, d, r2);
+    } else if constexpr (SCHEME == SCHEME_SR32) {
+        float f[5];
+        double ib2[3] = {0.0, 0.0, 0.0};
+        bool plateau = sr_features32(q, f);
+        closure_evaluated = !plateau;
+        if (!plateau) {
+            double br[3];
+            sr_ratios32(f, P, br);
+            sr_inv_sq<true>(br, ib2);
+        }
+        return sr_combine<true>(s, d, ib2, plateau, eno);
     } else {
         double f[5], ib2[3] = {0.0, 0.0, 0.0};
         bool plateau = sr_features(q, f);
@@ -533,6 +652,18 @@ __device__ __forceinline__ void recon_gauss(const double q[5], const MLP &P, int
         for (int k = 0; k < 3; ++k) { double r = tau5 / (b[k] + eps_z); r2[k] = r * r; }
         g1 = z_combine(s1, d1, r2);
         g2 = z_combine(s2, d2, r2);
+    } else if constexpr (SCHEME == SCHEME_SR32) {
+        float f[5];
+        double ib2[3] = {0.0, 0.0, 0.0};
+        bool plateau = sr_features32(q, f);
+        closure_evaluated = !plateau;
+        if (!plateau) {
+            double br[3];
+            sr_ratios32(f, P, br);
+            sr_inv_sq<true>(br, ib2);
+        }
+        g1 = sr_combine<true>(s1, d1, ib2, plateau, eno);
+        g2 = sr_combine<true>(s2, d2, ib2, plateau, eno);
     } else {
         double f[5], ib2[3] = {0.0, 0.0, 0.0};
         bool plateau = sr_features(q, f);
