@@ -154,14 +154,14 @@ def bench_shockbubble3d(args) -> int:
     import torch
     from weno_sr_publishing_amd.checkpoints import load_f32, load_f64
     from weno_sr_publishing_amd.euler3d import Euler3Solver, ShockBubbleConfig, primitive3
-    from weno_sr_publishing_amd.initial import make_shockbubble_bench_state
+    from weno_sr_publishing_amd.initial_device import shockbubble_bench_state
 
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}), flush=True)
         return 1
     dev = torch.device("cuda:0")
     steps = args.steps if args.steps > 0 else STEPS3
-    q_host = make_shockbubble_bench_state(NX3, NY3, NZ3)
+    q_init0 = shockbubble_bench_state(NX3, NY3, NZ3, device=dev)   # built on the device
     cfg = ShockBubbleConfig(nx=NX3, ny=NY3, nz=NZ3, x_end=float(NX3), y_end=float(NY3), z_end=float(NZ3),
                             cfl=0.4, t_end=float("inf"))
     mlp32 = load_f32()
@@ -173,7 +173,7 @@ def bench_shockbubble3d(args) -> int:
 
     def timed(scheme, instrument=False):
         s = make(scheme)
-        q_init = s.new_state(q_host)
+        q_init = s.new_state(q_init0)
         q = torch.empty_like(q_init)
         if args.warmup > 0:
             q.copy_(q_init)
@@ -205,7 +205,7 @@ def bench_shockbubble3d(args) -> int:
 
     s_cnt = make(args.closure)
     s_cnt.set_counting(True)
-    q_cnt = s_cnt.new_state(q_host)
+    q_cnt = s_cnt.new_state(q_init0)
     s_cnt.run(q_cnt, float("inf"), max_steps=steps, chunk=50)
     c = s_cnt.counters()
     frac_n, frac_g = c[1] / max(1, c[0]), c[3] / max(1, c[2])
@@ -277,7 +277,7 @@ def bench_shockbubble3d(args) -> int:
         threads = min(16, os.cpu_count() or 1)
         oracle.set_threads(threads)
         s_mid = make(args.closure)
-        q_mid = s_mid.new_state(q_host)
+        q_mid = s_mid.new_state(q_init0)
         s_mid.run(q_mid, float("inf"), max_steps=max(1, n_sr // 2), chunk=50)
         qm = q_mid.cpu().numpy()
         done_steps, t0 = 0, time.perf_counter()
@@ -332,14 +332,14 @@ def bench_doublemach7(args) -> int:
     from weno_sr_publishing_amd.checkpoints import WENO7_SR_FP64, load_weno7_checkpoint
     from weno_sr_publishing_amd.common import interior_stats
     from weno_sr_publishing_amd.euler7 import Euler7Solver, double_mach_params7
-    from weno_sr_publishing_amd.initial import make_double_mach_state
+    from weno_sr_publishing_amd.initial_device import double_mach_state
 
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no GPU visible"}), flush=True)
         return 1
     dev = torch.device("cuda:0")
     params = double_mach_params7(NX7, NY7, cfl=0.4, t_end=T7)
-    u_host = make_double_mach_state(NX7, NY7)
+    u_init0 = double_mach_state(NX7, NY7, device=dev)   # built on the device
     mlp64 = load_weno7_checkpoint(WENO7_SR_FP64)
     closure = {"sr32": "sr32", "sr64": "sr64", "srbf16": "sr32"}[args.closure]
     mlps = {"sr32": mlp64.as_precision("fp32"), "sr64": mlp64, "js": None}
@@ -351,7 +351,7 @@ def bench_doublemach7(args) -> int:
 
     def timed(scheme, instrument=False):
         s = make(scheme)
-        u_init = s.new_state(u_host)
+        u_init = s.new_state(u_init0)
         u = torch.empty_like(u_init)
         if args.warmup > 0:
             u.copy_(u_init)
@@ -387,7 +387,7 @@ def bench_doublemach7(args) -> int:
     _, _, _, _, wall_js_i, kms_js, kn_js = timed("js", instrument=True)
     s_cnt = make(closure)
     s_cnt.set_counting(True)
-    u_cnt = s_cnt.new_state(u_host)
+    u_cnt = s_cnt.new_state(u_init0)
     s_cnt.run(u_cnt, T7, max_steps=max_steps)
     c = s_cnt.counters()
     frac_n, frac_g = c[1] / max(1, c[0]), c[3] / max(1, c[2])
@@ -455,7 +455,7 @@ def bench_doublemach7(args) -> int:
         threads = min(16, os.cpu_count() or 1)
         oracle.set_threads(threads)
         s_mid = make(closure)
-        u_mid = s_mid.new_state(u_host)
+        u_mid = s_mid.new_state(u_init0)
         s_mid.run(u_mid, T7, max_steps=max(1, n_sr // 2))
         um = u_mid.cpu().numpy()
         done_steps, t0 = 0, time.perf_counter()
@@ -769,7 +769,7 @@ def main() -> int:
     from weno_sr_publishing_amd.checkpoints import load_f32, load_f64
     from weno_sr_publishing_amd.common import conservative_block_average, rho_errors, state_health
     from weno_sr_publishing_amd.euler2d import Euler2DSolver
-    from weno_sr_publishing_amd.initial import make_quadrant_initial
+    from weno_sr_publishing_amd.initial_device import quadrant_initial
     from weno_sr_publishing_amd.params import Params
 
     if not torch.cuda.is_available():
@@ -777,7 +777,7 @@ def main() -> int:
         return 1
     dev = torch.device("cuda:0")
     params = Params(nx=NX, ny=NY, x_length=1.0, y_length=1.0, cfl=0.4, t_end=T_END)
-    u0_host = make_quadrant_initial(NX, NY, "c4")
+    u0_dev = quadrant_initial(NX, NY, "c4", device=dev)   # built on the device
     mlp32 = load_f32()
     mlps = {"sr32": mlp32, "sr64": load_f64(), "srbf16": mlp32.as_precision("bf16"), "js": None}
 
@@ -822,7 +822,7 @@ def main() -> int:
         if loaded is not None:
             u = s.new_state(loaded[scheme])
         else:
-            u = s.new_state(u0_host)
+            u = s.new_state(u0_dev)
             if windowed:
                 s.run(u, t_mid)
         starts[scheme] = u
@@ -885,7 +885,7 @@ def main() -> int:
         full = {}
         for scheme in (args.closure, "js"):
             s = make(scheme)
-            u = s.new_state(u0_host)
+            u = s.new_state(u0_dev)
             n, _, w, _, _ = loop(s, u, T_END, 10_000_000, False)
             full[scheme] = (u, n, w)
     else:
@@ -951,7 +951,7 @@ def main() -> int:
     if not args.no_accuracy:
         p8 = Params(nx=2 * NX, ny=2 * NY, x_length=1.0, y_length=1.0, cfl=0.4, t_end=T_END)
         s8 = make("js", p8)
-        u8 = s8.new_state(make_quadrant_initial(2 * NX, 2 * NY, "c4"))
+        u8 = quadrant_initial(2 * NX, 2 * NY, "c4", device=dev)
         s8.run(u8, T_END)
         ref = conservative_block_average(u8.cpu().numpy()[3:-3, 3:-3], NX)[..., 0]
         acc = {}
@@ -960,7 +960,7 @@ def main() -> int:
                 field = full[scheme][0]
             else:
                 s = make(scheme)
-                field = s.new_state(u0_host)
+                field = s.new_state(u0_dev)
                 s.run(field, T_END)
             acc[scheme] = rho_errors(field.cpu().numpy()[3:-3, 3:-3, 0], ref)["rho_l1"]
         out["accuracy"] = {"rho_l1_vs_800sq_fp64_js": acc,

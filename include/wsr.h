@@ -318,6 +318,47 @@ int wsr_euler3_loop_steps_timed(const wsr_euler3_config *cfg, double *q, double 
 
 const char *wsr_version(void);
 
+/* ------------------------------------------------------------------------ */
+/* Initial states built on the device (cell averages by tensor-product       */
+/* Gauss-Legendre quadrature; nodes / weights are host arrays of nq <= 16     */
+/* values).  Each equals the host builder of weno_sr_publishing_amd/initial.py */
+/* bit for bit (the vortex up to the last bits of exp and pow).               */
+/* ------------------------------------------------------------------------ */
+
+/* Four-quadrant Riemann problem: states[16] = (rho, u, v, p) of the upper-right,
+ * upper-left, lower-left and lower-right quadrants (split at x = y = 0.5).  wrap = 0:
+ * every padded cell holds its own average (for_paper_results/run_quadrant.py:87-106
+ * make_quadrant_state); wrap = 1: the interior averages with the ghosts filled by
+ * periodic wrap (the WENO5 re-pad of run_weno5_timing.py:86-104 / run_vortex.py:34-35). */
+int wsr_init_quadrant(double *u, int32_t nx, int32_t ny, int32_t ghost, int32_t wrap, const double *states,
+                      double x_min, double x_max, double y_min, double y_max, double gamma, const double *nodes,
+                      const double *weights, int32_t nq, void *stream);
+
+/* Sod strip centred on 0 along axis 1 (x) or 2 (y), rho/p = 1/1 left, 0.125/0.1 right.
+ * Replaces teacherfree_lab_weno5/warp_sod_validation.py:176-210. */
+int wsr_init_sod(double *u, int32_t nx, int32_t ny, int32_t ghost, int32_t axis, double x_length, double y_length,
+                 double gamma, const double *nodes, const double *weights, int32_t nq, void *stream);
+
+/* Double-Mach reflection at t = 0 (Mach-10 shock through x = 1/6 at 60 degrees).
+ * Replaces run_double_mach_compare.py:69-88 make_initial_state. */
+int wsr_init_double_mach(double *u, int32_t nx, int32_t ny, int32_t ghost, double x_length, double y_length,
+                         double gamma, const double *nodes, const double *weights, int32_t nq, void *stream);
+
+/* Periodic isentropic vortex (strength 5, advected with (1, 1), period 20) at time t.
+ * Replaces weno7_point_rk4_shu/point_rk4.py:102-117 cell_average_state. */
+int wsr_init_vortex(double *u, int32_t nx, int32_t ny, int32_t ghost, double t, double x_min, double x_max,
+                    double y_min, double y_max, double gamma, const double *nodes, const double *weights, int32_t nq,
+                    void *stream);
+
+/* 3-D shock-bubble, state (nz+2g, ny+2g, nx+2g, 5).  kind 0: the unit-spacing
+ * benchmark case (shock at x < 4, bubble of radius ny/4 centred at (ny/2 + 8, ny/2,
+ * nz/2) with density density_ratio; lengths = (nx, ny, nz)); kind 1: Ma = 3 in SI
+ * units (warp_weno5_3d_rk3/kernels.py:730-799; density_ratio = helium density).
+ * shock[3] = post-shock (rho, u, p). */
+int wsr_init_shockbubble3d(double *q, int32_t nx, int32_t ny, int32_t nz, int32_t ghost, int32_t kind,
+                           const double *lengths, const double *shock, double density_ratio, double gamma,
+                           const double *nodes, const double *weights, int32_t nq, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,3 +65,21 @@ def test_adapters_reject_cpu_device():
     p = Params(nx=4, ny=4)
     with pytest.raises(ValueError):
         _adapter.allocate_forward(np.zeros(p.padded_shape), p, "cpu")
+
+
+def test_initial_state_argument_checks():
+    """The device initial-state builders refuse bad arguments before any launch."""
+    L = _lib.lib()
+    states = (ctypes.c_double * 16)(*([1.0] * 16))
+    nodes = (ctypes.c_double * 2)(-0.5, 0.5)
+    weights = (ctypes.c_double * 2)(1.0, 1.0)
+    a = 1 << 12
+    assert L.wsr_init_quadrant(None, 8, 8, 3, 1, states, 0.0, 1.0, 0.0, 1.0, 1.4, nodes, weights, 2, None) == -1
+    assert L.wsr_init_quadrant(a, 0, 8, 3, 1, states, 0.0, 1.0, 0.0, 1.0, 1.4, nodes, weights, 2, None) == -1
+    # quadrature orders 1..16 only
+    assert L.wsr_init_quadrant(a, 8, 8, 3, 1, states, 0.0, 1.0, 0.0, 1.0, 1.4, nodes, weights, 17, None) == -1
+    assert L.wsr_init_sod(a, 8, 8, 3, 3, 1.0, 1.0, 1.4, nodes, weights, 2, None) == -1
+    assert L.wsr_init_double_mach(a, 8, 8, 4, 4.0, 1.0, 1.4, None, weights, 2, None) == -1
+    assert L.wsr_init_vortex(a, 8, -1, 4, 0.0, -10.0, 10.0, -10.0, 10.0, 1.4, nodes, weights, 2, None) == -1
+    lengths = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
+    assert L.wsr_init_shockbubble3d(a, 8, 8, 8, 3, 2, lengths, lengths, 0.1, 1.4, nodes, weights, 2, None) == -1

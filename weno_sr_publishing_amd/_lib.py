@@ -25,6 +25,7 @@ EXPORTS = (
     "wsr_euler7_loop_steps", "wsr_euler7_loop_steps_timed",
     "wsr_euler3_workspace_bytes", "wsr_euler3_step", "wsr_euler3_max_speed", "wsr_euler3_loop_init",
     "wsr_euler3_loop_steps", "wsr_euler3_loop_steps_timed",
+    "wsr_init_quadrant", "wsr_init_sod", "wsr_init_double_mach", "wsr_init_vortex", "wsr_init_shockbubble3d",
 )
 FLAG_COUNT_CLOSURE = 1
 FLAG_Z_FALLBACK = 2
@@ -141,6 +142,13 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler7_loop_steps_timed.restype = ctypes.c_int
         L.wsr_euler7_loop_steps_timed.argtypes = [C7, P, P, I32, P, SZ, P, ctypes.POINTER(D), ctypes.POINTER(I32)]
         C3 = ctypes.POINTER(Euler3Config)
+        for name, args in (("wsr_init_quadrant", [P, I32, I32, I32, I32, P, D, D, D, D, D, P, P, I32, P]),
+                           ("wsr_init_sod", [P, I32, I32, I32, I32, D, D, D, P, P, I32, P]),
+                           ("wsr_init_double_mach", [P, I32, I32, I32, D, D, D, P, P, I32, P]),
+                           ("wsr_init_vortex", [P, I32, I32, I32, D, D, D, D, D, D, P, P, I32, P]),
+                           ("wsr_init_shockbubble3d", [P, I32, I32, I32, I32, I32, P, P, D, D, P, P, I32, P])):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = args
         L.wsr_euler3_workspace_bytes.restype = SZ
         L.wsr_euler3_workspace_bytes.argtypes = [C3]
         L.wsr_euler3_step.restype = ctypes.c_int

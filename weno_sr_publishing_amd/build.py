@@ -19,7 +19,9 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libwsr_hip.so"
 OBJ_DIR = PKG / "_build"
 SOURCES = [CSRC / "wsr_euler2d.hip", CSRC / "wsr_euler7.hip", CSRC / "wsr_euler3.hip", CSRC / "wsr_adv1d.hip",
-           CSRC / "wsr_train1d.hip"]
+           CSRC / "wsr_train1d.hip", CSRC / "wsr_init.hip"]
+# the initial-state kernels reproduce the host builders' roundings: no contraction
+SOURCE_FLAGS = {"wsr_init.hip": ["-ffp-contract=off"]}
 HEADERS = [CSRC / "wsr_math.hpp", CSRC / "wsr_loop.hpp", CSRC / "wsr_weno7.hpp", CSRC / "wsr_adv1d.hpp",
            CSRC / "wsr_adjoint.hpp", PKG.parent / "include" / "wsr.h"]
 ARCH = os.environ.get("WSR_OFFLOAD_ARCH", "gfx950")
@@ -55,7 +57,7 @@ def build(verbose: bool = False, extra: list[str] | None = None) -> Path:
 
     def compile_one(src: Path) -> Path:
         obj = OBJ_DIR / (src.stem + ".o")
-        cmd = [cc, *HIPCC_FLAGS, *extra, "-c", "-o", str(obj), str(src)]
+        cmd = [cc, *HIPCC_FLAGS, *SOURCE_FLAGS.get(src.name, []), *extra, "-c", "-o", str(obj), str(src)]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

@@ -26,6 +26,16 @@ from .params import Params
 DEFAULT_CHUNK = 64
 
 
+def _state_copy(state, shape: tuple, device: torch.device) -> torch.Tensor:
+    """A contiguous FP64 copy of ``state`` (host array or torch tensor) on ``device``,
+    checked against the padded shape."""
+    if tuple(state.shape) != tuple(shape):
+        raise ValueError(f"initial state must have shape {tuple(shape)}, got {tuple(state.shape)}")
+    if isinstance(state, torch.Tensor):
+        return state.detach().to(device=device, dtype=torch.float64, copy=True).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(state, dtype=np.float64)).to(device)
+
+
 def scheme_for(mlp: MlpParams | None, closure: str | None = None) -> str:
     """Closure name from the MLP precision ("js" without parameters)."""
     if closure is not None:
@@ -92,13 +102,12 @@ class Euler2DSolver:
         if not u.is_contiguous() or u.device != self.device:
             raise ValueError("state must be a contiguous tensor on the solver's device")
 
-    def new_state(self, host: np.ndarray | None = None) -> torch.Tensor:
+    def new_state(self, host: np.ndarray | torch.Tensor | None = None) -> torch.Tensor:
+        """A fresh solver state: zeros, a copy of a host array, or a device copy of a
+        torch tensor (the device initial states of ``initial_device``)."""
         if host is None:
             return torch.zeros(self.params.padded_shape, dtype=torch.float64, device=self.device)
-        host = np.ascontiguousarray(host, dtype=np.float64)
-        if host.shape != self.params.padded_shape:
-            raise ValueError(f"initial state must have shape {self.params.padded_shape}, got {host.shape}")
-        return torch.from_numpy(host).to(self.device)
+        return _state_copy(host, self.params.padded_shape, self.device)
 
     # ------------------------------------------------------------ one stage
     def stage(self, u_stage: torch.Tensor, u0: torch.Tensor, u_out: torch.Tensor, dt: float, rk: int,

@@ -24,6 +24,7 @@ import torch
 
 from . import _lib
 from .checkpoints import MlpParams
+from .euler2d import _state_copy
 
 DEFAULT_CHUNK = 32
 SCHEMES3 = ("js", "sr64", "sr32", "srbf16")
@@ -135,10 +136,10 @@ class Euler3Solver:
     def _stream(self) -> int:
         return torch.cuda.current_stream(self.device).cuda_stream
 
-    def new_state(self, host: np.ndarray) -> torch.Tensor:
-        if tuple(host.shape) != self.config.padded_shape:
-            raise ValueError(f"state shape {host.shape} != {self.config.padded_shape}")
-        return torch.tensor(np.ascontiguousarray(host, dtype=np.float64), device=self.device)
+    def new_state(self, host: np.ndarray | torch.Tensor) -> torch.Tensor:
+        """A solver state copied from a host array or a torch tensor (the device
+        initial states of ``initial_device``)."""
+        return _state_copy(host, self.config.padded_shape, self.device)
 
     def _check(self, q: torch.Tensor) -> None:
         if q.dtype != torch.float64 or not q.is_contiguous() or tuple(q.shape) != self.config.padded_shape:
@@ -147,14 +148,17 @@ class Euler3Solver:
             raise ValueError("state on the wrong device")
 
     # ------------------------------------------------------------- reference-style API
-    def initialize(self, host_state: np.ndarray | None = None) -> None:
-        """solver.py:134-150: the Ma = 3 shock-bubble cell averages (or a given state)."""
+    def initialize(self, host_state: np.ndarray | torch.Tensor | None = None) -> None:
+        """solver.py:134-150: the Ma = 3 shock-bubble cell averages, built on the device
+        (or a given state)."""
         if host_state is None:
-            from .initial import make_shockbubble_ma3_state
+            from .initial_device import shockbubble_ma3_state
             c = self.config
-            host_state = make_shockbubble_ma3_state(c.nx, c.ny, c.nz, c.ghost,
-                                                    (c.x_end - c.x_start, c.y_end - c.y_start, c.z_end - c.z_start))
-        self.q = self.new_state(host_state)
+            self.q = shockbubble_ma3_state(c.nx, c.ny, c.nz, c.ghost,
+                                           (c.x_end - c.x_start, c.y_end - c.y_start, c.z_end - c.z_start),
+                                           device=self.device)
+        else:
+            self.q = self.new_state(host_state)
         self.time = float(self.config.t_start)
         self.step_count = 0
         self.dt_trace = []

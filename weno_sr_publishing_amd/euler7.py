@@ -23,6 +23,7 @@ import torch
 from . import _lib
 from .checkpoints import Mlp7Params
 from .common import interior_stats
+from .euler2d import _state_copy
 
 DEFAULT_CHUNK = 64
 
@@ -119,10 +120,10 @@ class Euler7Solver:
     def _stream(self) -> int:
         return torch.cuda.current_stream(self.device).cuda_stream
 
-    def new_state(self, host: np.ndarray) -> torch.Tensor:
-        if tuple(host.shape) != self.params.padded_shape:
-            raise ValueError(f"state shape {host.shape} != {self.params.padded_shape}")
-        return torch.tensor(np.ascontiguousarray(host, dtype=np.float64), device=self.device)
+    def new_state(self, host: np.ndarray | torch.Tensor) -> torch.Tensor:
+        """A solver state copied from a host array or a torch tensor (the device
+        initial states of ``initial_device``)."""
+        return _state_copy(host, self.params.padded_shape, self.device)
 
     def _check(self, u: torch.Tensor) -> None:
         if u.dtype != torch.float64 or not u.is_contiguous() or tuple(u.shape) != self.params.padded_shape:

@@ -15,11 +15,13 @@ import argparse
 import json
 
 import numpy as np
+import torch
 
 from .. import checkpoints
 from ..common import interior, state_health
 from ..euler7 import Euler7Solver
 from ..initial import make_double_mach_state
+from ..initial_device import double_mach_state
 from ..solvers import euler_methods, weno5_hllc_mixed, weno5_hllc_refsym
 from ..solvers.point_rk4_mlp import TorchWeno7PointBeta
 from ..solvers import weno7_double_mach
@@ -30,12 +32,20 @@ METHODS = ("weno5_js", "weno5_z_p2", "weno5_sr_f64", "weno5_sr_f32", "weno7_js",
 CUT_Y = (0.2, 0.4, 0.6)
 
 
+def _initial(nx: int, ny: int, ghost: int, device) -> np.ndarray:
+    """run_double_mach_compare.py:69-88 cell averages, computed on a GPU ``device``
+    (equal bit for bit to ``initial.make_double_mach_state``)."""
+    if torch.device(device).type == "cuda":
+        return double_mach_state(nx, ny, ghost, device=device).cpu().numpy()
+    return make_double_mach_state(nx, ny, ghost=ghost)
+
+
 def run_method(key: str, nx: int = 1200, ny: int = 300, cfl: float = 0.4, t_end: float = 0.2,
                device: str = "cuda", report_interval: int = 0, z7_unit_gauss_h: bool = False):
     """(padded final state, dts, steps, t, ghost) of one method."""
     if key.startswith("weno5"):
         params = euler_methods.make_weno5_params(nx, ny, 4.0, 1.0, cfl, t_end, key == "weno5_sr_f32")
-        initial = make_double_mach_state(nx, ny, ghost=3)
+        initial = _initial(nx, ny, 3, device)
         from ..solvers import _adapter
         if key == "weno5_js":
             # the reference's WENO5-JS field comes from run_double_mach_compare.py (its
@@ -54,7 +64,7 @@ def run_method(key: str, nx: int = 1200, ny: int = 300, cfl: float = 0.4, t_end:
         final, dts, steps, t = out
         return final, dts, steps, t, 3
     params = weno7_double_mach.Params(nx=nx, ny=ny, x_min=0.0, x_max=4.0, y_min=0.0, y_max=1.0, cfl=cfl, t_end=t_end)
-    initial = make_double_mach_state(nx, ny, ghost=4)
+    initial = _initial(nx, ny, 4, device)
     if key == "weno7_z_p3":
         solver = Euler7Solver(params, "z", "hllc", "double-mach", loop="double-mach", device=device,
                               unit_gauss_h=z7_unit_gauss_h)

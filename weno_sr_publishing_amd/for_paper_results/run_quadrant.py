@@ -15,6 +15,7 @@ import json
 import time
 
 import numpy as np
+import torch
 
 from ..common import interior, state_health
 from ..initial import CASES, make_quadrant_state as _quadrant_cells, pad_periodic
@@ -25,8 +26,15 @@ from ._io import write_json
 Z_METHOD = "weno7_z_p3"
 
 
-def make_quadrant_state(params, definition: dict, quadrature: int) -> np.ndarray:
-    """Cell averages on the padded grid of ``params`` (run_quadrant.py:87-103)."""
+def make_quadrant_state(params, definition: dict, quadrature: int, device: str | None = None) -> np.ndarray:
+    """Cell averages on the padded grid of ``params`` (run_quadrant.py:87-103).  With a
+    GPU ``device`` they are computed there (``initial_device.quadrant_state``, equal bit
+    for bit) and returned as a host array."""
+    if device is not None and torch.device(device).type == "cuda":
+        from ..initial_device import quadrant_state
+        state = quadrant_state(params.nx, params.ny, params.ghost, definition, quadrature, params.x_min,
+                               params.x_max, params.y_min, params.y_max, params.gamma, device=device)
+        return state.cpu().numpy()
     return _quadrant_cells(params.nx, params.ny, params.ghost, definition, quadrature, params.x_min,
                            params.x_max, params.y_min, params.y_max, params.gamma)
 
@@ -39,7 +47,7 @@ def run_method(key: str, case: str, nx: int, ny: int, cfl: float, t_end: float, 
     recorded results (see ``solvers.euler_z.run_weno7``)."""
     definition = CASES[case]
     p7 = euler_methods.make_weno7_params(nx, ny, 0.0, 1.0, 0.0, 1.0, cfl, t_end)
-    initial7 = make_quadrant_state(p7, definition, init_quadrature)
+    initial7 = make_quadrant_state(p7, definition, init_quadrature, device)
     started = time.perf_counter()
     if key.startswith("weno5"):
         params = euler_methods.make_weno5_params(nx, ny, 1.0, 1.0, cfl, t_end, key == "weno5_sr_f32")

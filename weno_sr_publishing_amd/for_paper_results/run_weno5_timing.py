@@ -57,7 +57,7 @@ def run(repeats: int = 3, device: str = "cuda", methods=("weno5_js", "weno5_sr_f
     case, case_label, definition = "c4", "C.4", CASES["c4"]
     t_end = float(definition["t_end"])
     p7 = euler_methods.make_weno7_params(nx, nx, 0.0, 1.0, 0.0, 1.0, 0.4, t_end)
-    initial_interior = interior(make_quadrant_state(p7, definition, 15), p7.ghost, nx, nx)
+    initial_interior = interior(make_quadrant_state(p7, definition, 15, device), p7.ghost, nx, nx)
     rows = []
     for key, label, mixed, adapter, precision in VARIANTS:
         if key not in selected:
