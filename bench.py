@@ -44,18 +44,45 @@ REFERENCE_SR32_WALL_S = 14.253   # experiments/14_mixed_precision_timing/tables/
 REFERENCE_JS_WALL_S = 6.605      # same table, line 9
 
 # ---------------------------------------------------------------- flop model
-# Algorithmic operations per unit (add/sub/mul/div/sqrt/exp/log/tanh = 1 each;
-# comparisons, min/max/abs and sign flips = 0), counted from the reference
-# expressions (warp_weno5_helpers.py, weno5_rk3_diff_mlp_f32.py, weno5_hllc.py):
-MLP_RAW = 415            # 5-10-6-6-3 MLP: 346 affine + 66 swish + 9 tanh head + 12 softmax
-SR_CLOSURE = 2 * MLP_RAW + 25 + 6 + 12   # + non-plateau features, reflection average, 1/(3r+eps)^2
-FEAT_ALWAYS = 28         # second differences, deltas, plateau test
-NORMAL = {"js": 70, "z": 75,
-          "sr": (FEAT_ALWAYS + 15 + 5, FEAT_ALWAYS + 15 + 14 + SR_CLOSURE)}   # (plateau, closure)
-GAUSS = {"js": 101, "z": 106,
-         "sr": (FEAT_ALWAYS + 30 + 10, FEAT_ALWAYS + 30 + 28 + SR_CLOSURE)}
-FACE_LINE_FIXED = 68 + 42 + 40 * 8 + 8 * 7   # Roe + eigenvalues, left rows, 40 projections, back-projection
-FLUX_FACE_FIXED = 2 * 110 + 12               # two HLLC fluxes + dt/h scaling
+# Operations per call of each building block of the WENO5 step, as tallied by the
+# op-counting build of the oracle (a counting scalar over the unchanged oracle source:
+# add/sub, mul, div, sqrt, exp, log and tanh count one each; comparisons, min/max, fabs
+# and sign flips zero; tests/test_opcount.py checks each value against the tally):
+OPS = {
+    "roe": 51, "jacobian": 17,              # Roe average; its (u, v, a, H)
+    "project": 53, "back_project": 30,      # one cell onto the 4 fields; one face back
+    "candidates_normal": 15, "candidates_gauss": 14,
+    "js_beta": 51, "js_combine": 22, "z_combine": 27,
+    "raw_sensors": 42, "plateau_test": 1,   # SR: deltas and gamma_s; the plateau compare
+    "features": 9, "mlp_raw": 415,          # SR: features beyond the sensors; one 5-10-6-6-3 pass
+    "reflection_average": 6, "sr_weights": 21, "weighted_sum": 5,
+    "hllc": 104,                            # subsonic (star-state) path
+}
+MLP_RAW = OPS["mlp_raw"]
+# The kernel's multiplicities: what the algorithm needs once, counted once.  A normal
+# reconstruction is one head of one stencil; a Gauss pair is one transverse stencil
+# with both Gauss-point heads, sharing its smoothness indicators (JS, Z) or its
+# plateau test, features and closure (SR); the reflected features are a permutation
+# of the direct ones.
+_SR_SHARED = OPS["raw_sensors"] + OPS["plateau_test"]
+_SR_CLOSURE = OPS["features"] + 2 * OPS["mlp_raw"] + OPS["reflection_average"]
+NORMAL = {
+    "js": OPS["candidates_normal"] + OPS["js_beta"] + OPS["js_combine"],
+    "z": OPS["candidates_normal"] + OPS["js_beta"] + OPS["z_combine"],
+    "sr": (_SR_SHARED + OPS["candidates_normal"] + OPS["weighted_sum"],                       # plateau
+           _SR_SHARED + OPS["candidates_normal"] + OPS["weighted_sum"] + _SR_CLOSURE + OPS["sr_weights"]),
+}
+GAUSS = {
+    "js": OPS["js_beta"] + 2 * (OPS["candidates_gauss"] + OPS["js_combine"]),
+    "z": OPS["js_beta"] + 2 * (OPS["candidates_gauss"] + OPS["z_combine"]),
+    "sr": (_SR_SHARED + 2 * (OPS["candidates_gauss"] + OPS["weighted_sum"]),
+           _SR_SHARED + 2 * (OPS["candidates_gauss"] + OPS["weighted_sum"]) + _SR_CLOSURE + 2 * OPS["sr_weights"]),
+}
+# per face and transverse line: one Roe average and its Jacobian values, the six cells
+# of the two stencils projected, the two face states projected back
+FACE_LINE_FIXED = OPS["roe"] + OPS["jacobian"] + 6 * OPS["project"] + 2 * OPS["back_project"]
+# per face: the two Gauss-point HLLC fluxes and their dt/h scaling
+FLUX_FACE_FIXED = 2 * OPS["hllc"] + 8
 BYTES_PER_SWEEP_CELL = 32 + 32               # state tile read + flux write (FP64 x 4)
 # HBM bytes per SR32 sweep launch at 400^2 (one launch = the x and the y tiles), from
 # the two rocprofv3 counter passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000 after the
@@ -301,26 +328,52 @@ def bench_shockbubble3d(args) -> int:
 
 # ------------------------------------------------------------- WENO7 workload
 # Double Mach reflection on [0,4]x[0,1] at 1200x300 (Mach-10 shock at 60 degrees,
-# pre-shock rho=1.4, p=1), WENO7-SR with the FP32 closure and per-reconstruction
+# pre-shock rho=1.4, p=1), WENO7-SR with the FP32 closure and the state-level
 # WENO7-Z fallback, HLLC, Shu RK4 with the downwind operator, CFL 0.4, t=0.2.
 NX7, NY7, T7 = 1200, 300, 0.2
-# algorithmic operations of one WENO7 scalar reconstruction (same counting rules)
-MLP7_RAW = 1952                           # 6-24-16-16-4: 1756 affine + 168 swish + 12 tanh head + 16 softmax
-SR7_CLOSURE = 2 * MLP7_RAW + 35 + 8 + 23  # + non-plateau features, reflection average, weights (beta = 4 r)
-FEAT7_ALWAYS = 50                         # four normalised deltas, plateau test
-NORMAL7 = {"js": 150, "z": 160, "sr": (28 + FEAT7_ALWAYS + 7, 28 + FEAT7_ALWAYS + 7 + SR7_CLOSURE)}
-GAUSS7 = {"js": 204, "z": 214, "sr": (56 + FEAT7_ALWAYS + 14, 56 + FEAT7_ALWAYS + 14 + SR7_CLOSURE)}
-FACE_LINE7_FIXED = 68 + 8 * (10 + 56 + 7)  # Roe + eigen; per lane: left row, 7 projections, back-projection
+# Operations per call of the building blocks of a WENO7 scalar reconstruction, tallied
+# by the op-counting build of the oracle (tests/test_opcount.py checks each value):
+OPS7 = {
+    "candidates_normal": 28, "candidates_gauss": 64,
+    "beta_poly": 116, "js_combine": 35,      # JS: smoothness indicators; weights and sum of one head
+    "beta_sos": 96, "z_combine": 43,         # Z: the same
+    "plateau_test": 85, "features": 132,     # SR: plateau test (always), features (closure stencils)
+    "mlp_raw": 2008,                         # one 6-24-16-16-4 pass
+    "reflection_average": 8, "beta_from_ratios": 4, "sr_combine": 30,
+    "guard": 121,                            # Z-fallback check of one head: beta_sos + the trip test
+}
+MLP7_RAW = OPS7["mlp_raw"]
+_TRIP7 = OPS7["guard"] - OPS7["beta_sos"]
+_SR7_CLOSURE = OPS7["features"] + 2 * OPS7["mlp_raw"] + OPS7["reflection_average"] + OPS7["beta_from_ratios"]
+NORMAL7 = {
+    "js": OPS7["candidates_normal"] + OPS7["beta_poly"] + OPS7["js_combine"],
+    "z": OPS7["candidates_normal"] + OPS7["beta_sos"] + OPS7["z_combine"],
+    "sr": (OPS7["plateau_test"] + OPS7["candidates_normal"] + OPS7["sr_combine"],
+           OPS7["plateau_test"] + OPS7["candidates_normal"] + OPS7["sr_combine"] + _SR7_CLOSURE),
+}
+# a Gauss pair: one transverse stencil, both Gauss-point heads, sharing the smoothness
+# indicators or the plateau test and closure (and the guard's beta_sos)
+GAUSS7 = {
+    "js": OPS7["beta_poly"] + 2 * (OPS7["candidates_gauss"] + OPS7["js_combine"]),
+    "z": OPS7["beta_sos"] + 2 * (OPS7["candidates_gauss"] + OPS7["z_combine"]),
+    "sr": (OPS7["plateau_test"] + 2 * (OPS7["candidates_gauss"] + OPS7["sr_combine"]),
+           OPS7["plateau_test"] + 2 * (OPS7["candidates_gauss"] + OPS7["sr_combine"]) + _SR7_CLOSURE),
+}
+GUARD7 = {"normal": OPS7["guard"], "gauss": OPS7["beta_sos"] + 2 * _TRIP7}
+# per face and transverse line: Roe average and Jacobian values, the eight cells of the
+# two 7-cell stencils projected, the two face states projected back
+FACE_LINE7_FIXED = OPS["roe"] + OPS["jacobian"] + 8 * OPS["project"] + 2 * OPS["back_project"]
 
 
-def sweep7_flops_per_launch(scheme: str, frac_n: float, frac_g: float, both: bool) -> float:
-    """One k7_sweep_xy launch at 1200x300: the x tiles and the y tiles."""
+def sweep7_flops_per_launch(scheme: str, frac_n: float, frac_g: float, both: bool, guard: bool = False) -> float:
+    """One k7_sweep_xy launch at 1200x300: the x tiles and the y tiles; ``both`` adds
+    the downwind operator's fluxes, ``guard`` the Z-fallback check of closure stencils."""
     if scheme.startswith("sr"):
-        n = NORMAL7["sr"][0] + frac_n * (NORMAL7["sr"][1] - NORMAL7["sr"][0])
-        g = GAUSS7["sr"][0] + frac_g * (GAUSS7["sr"][1] - GAUSS7["sr"][0])
+        n = NORMAL7["sr"][0] + frac_n * (NORMAL7["sr"][1] - NORMAL7["sr"][0] + (GUARD7["normal"] if guard else 0))
+        g = GAUSS7["sr"][0] + frac_g * (GAUSS7["sr"][1] - GAUSS7["sr"][0] + (GUARD7["gauss"] if guard else 0))
     else:
         n, g = NORMAL7[scheme], GAUSS7[scheme]
-    flux = (4 if both else 2) * 110 + (8 if both else 4)
+    flux = (4 if both else 2) * OPS["hllc"] + (8 if both else 4)
     total = 0.0
     for nfaces, ntrans in ((NX7 + 1, NY7), (NY7 + 1, NX7)):
         total += nfaces * (ntrans + 6) * (FACE_LINE7_FIXED + 8 * n) + nfaces * ntrans * (8 * g + flux)
@@ -392,7 +445,7 @@ def bench_doublemach7(args) -> int:
     c = s_cnt.counters()
     frac_n, frac_g = c[1] / max(1, c[0]), c[3] / max(1, c[2])
     both_ms = kms[0] / max(1, kn[0])
-    flops = sweep7_flops_per_launch(closure, frac_n, frac_g, True)
+    flops = sweep7_flops_per_launch(closure, frac_n, frac_g, True, guard=True)
     achieved = flops / (both_ms * 1e-3) / 1e12
     js_both_ms = kms_js[0] / max(1, kn_js[0])
     js_achieved = sweep7_flops_per_launch("js", 0.0, 0.0, True) / (js_both_ms * 1e-3) / 1e12

@@ -29,7 +29,10 @@ _lib = None
 
 def build(force: bool = False) -> Path:
     """Compile the oracle with its Makefile (no FMA contraction, no fast-math)."""
-    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "wsr_oracle.c").stat().st_mtime:
+    opc = HERE / "libwsr_opcount.so"
+    stale = not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < (HERE / "wsr_oracle.c").stat().st_mtime
+    stale = stale or not opc.exists() or opc.stat().st_mtime < (HERE / "opcount.cpp").stat().st_mtime
+    if force or stale:
         subprocess.run(["make", "-s", "-C", str(HERE), "-B" if force else "all"], check=True)
     return LIB_PATH
 

@@ -317,11 +317,8 @@ static void js_beta(const double q[5], double beta[3]) {
 /* H5:241-242 safe_rcp */
 static double safe_rcp(double x) { return 1.0 / (1.0e-6 + x); }
 
-static double weno5_js_value(const double q[5], int head) {
-    double s[3], d[3], beta[3];
-    candidates(q, head, s);
-    optimal_weights(head, d);
-    js_beta(q, beta);
+/* JS weights of one head from the smoothness indicators, and the weighted sum */
+static double js_combine(const double s[3], const double d[3], const double beta[3]) {
     double inv0 = safe_rcp(beta[0]), inv1 = safe_rcp(beta[1]), inv2 = safe_rcp(beta[2]);
     double alpha0 = d[0] * inv0 * inv0;
     double alpha1 = d[1] * inv1 * inv1;
@@ -331,12 +328,16 @@ static double weno5_js_value(const double q[5], int head) {
     return w0 * s[0] + w1 * s[1] + w2 * s[2];
 }
 
-/* WZ:511-570 (face) and WZ:818-880 (Gauss): WENO5-Z, tau5 = |b0-b2|, eps = h^2, p = 2 */
-static double weno5_z_value(const double q[5], int head, double h) {
+static double weno5_js_value(const double q[5], int head) {
     double s[3], d[3], beta[3];
     candidates(q, head, s);
     optimal_weights(head, d);
     js_beta(q, beta);
+    return js_combine(s, d, beta);
+}
+
+/* WZ:511-570 (face) and WZ:818-880 (Gauss): WENO5-Z, tau5 = |b0-b2|, eps = h^2, p = 2 */
+static double z_combine(const double s[3], const double d[3], const double beta[3], double h) {
     double tau5 = fabs(beta[0] - beta[2]);
     double eps_z = h * h;
     double ratio0 = tau5 / (beta[0] + eps_z);
@@ -348,6 +349,14 @@ static double weno5_z_value(const double q[5], int head, double h) {
     double alpha_sum = alpha0 + alpha1 + alpha2;
     double w0 = alpha0 / alpha_sum, w1 = alpha1 / alpha_sum, w2 = alpha2 / alpha_sum;
     return w0 * s[0] + w1 * s[1] + w2 * s[2];
+}
+
+static double weno5_z_value(const double q[5], int head, double h) {
+    double s[3], d[3], beta[3];
+    candidates(q, head, s);
+    optimal_weights(head, d);
+    js_beta(q, beta);
+    return z_combine(s, d, beta, h);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -537,22 +546,24 @@ static void raw_ratios(const double f[5], int scheme, const void *mlp, double r[
 }
 
 /* C64:209-281 / C32:206-283 mlp_weights: reflection average 0.5[M(x) + P3 M(P5 x)] */
-static void mlp_weights(const double q[5], int head, int scheme, const void *mlp, int eno_cutoff,
-                        double w[3]) {
-    double c[3];
-    optimal_weights(head, c);
-    if (plateau_detected(q[0], q[1], q[2], q[3], q[4])) {
-        w[0] = c[0]; w[1] = c[1]; w[2] = c[2];
-        return;
-    }
+/* the part shared by every head of one stencil: plateau test, features of the stencil
+ * and of its reflection, the two raw evaluations and their average; 0 on a plateau */
+static int sr_ratios(const double q[5], int scheme, const void *mlp, double br[3]) {
+    if (plateau_detected(q[0], q[1], q[2], q[3], q[4])) return 0;
     double fd[5], fr[5], direct[3], reflected[3];
     nn_features(q[0], q[1], q[2], q[3], q[4], fd);
     nn_features(q[4], q[3], q[2], q[1], q[0], fr);
     raw_ratios(fd, scheme, mlp, direct);
     raw_ratios(fr, scheme, mlp, reflected);
-    double br0 = 0.5 * (direct[0] + reflected[2]);
-    double br1 = 0.5 * (direct[1] + reflected[1]);
-    double br2 = 0.5 * (direct[2] + reflected[0]);
+    br[0] = 0.5 * (direct[0] + reflected[2]);
+    br[1] = 0.5 * (direct[1] + reflected[1]);
+    br[2] = 0.5 * (direct[2] + reflected[0]);
+    return 1;
+}
+
+/* the weights of one head from the averaged ratios */
+static void sr_weights(const double br[3], const double c[3], int eno_cutoff, double w[3]) {
+    double br0 = br[0], br1 = br[1], br2 = br[2];
     double eps_ratio = 1.0e-12;
     double ratio_scale = 3.0;
     double sb0 = ratio_scale * br0, sb1 = ratio_scale * br1, sb2 = ratio_scale * br2;
@@ -574,6 +585,17 @@ static void mlp_weights(const double q[5], int head, int scheme, const void *mlp
         w2 = psi2 * w2 * inv_cutoff_sum;
     }
     w[0] = w0; w[1] = w1; w[2] = w2;
+}
+
+static void mlp_weights(const double q[5], int head, int scheme, const void *mlp, int eno_cutoff,
+                        double w[3]) {
+    double c[3], br[3];
+    optimal_weights(head, c);
+    if (!sr_ratios(q, scheme, mlp, br)) {
+        w[0] = c[0]; w[1] = c[1]; w[2] = c[2];
+        return;
+    }
+    sr_weights(br, c, eno_cutoff, w);
 }
 
 /* C64:299-334 weno5_lr_value_mlp / C64:337-372 weno5_gauss_value_mlp */
