@@ -448,12 +448,34 @@ __device__ __forceinline__ void softmax_head_folded(const float u[3], float r[3]
     r[0] = e[0] * inv; r[1] = e[1] * inv; r[2] = e[2] * inv;
 }
 
-__device__ __forceinline__ void mlp_pair32_folded(const float xa[5], const float xb[5], const float *w, float ra[3],
-                                                  float rb[3]) {
+// First layer of the reflected pair: the reflected input (x2, x1, x0, x3, x4) shares
+// x1, x3 and x4 with the direct one, so each neuron sums the bias and those three
+// terms once and adds the two remaining products per input (7 FMAs for the pair
+// instead of 10).
+template <class ACT>
+__device__ __forceinline__ void dense_pair_reflected5(const float x[5], const float *w, float *ha, float *hb, ACT act) {
+#pragma unroll
+    for (int o = 0; o < 10; ++o) {
+        kconst_ptr<float> c = kparams(w + DEV_L1 + o * 6);   // one neuron's run at a time
+        const int b = 0;
+        float shared = c[b] + x[1] * c[b + 2];
+        shared = shared + x[3] * c[b + 4];
+        shared = shared + x[4] * c[b + 5];
+        float za = shared + x[0] * c[b + 1];
+        za = za + x[2] * c[b + 3];
+        float zb = shared + x[2] * c[b + 1];
+        zb = zb + x[0] * c[b + 3];
+        ha[o] = act(za);
+        hb[o] = act(zb);
+    }
+}
+
+// ra = M(x), rb = M(P5 x) with P5 x = (x2, x1, x0, x3, x4).
+__device__ __forceinline__ void mlp_pair32_folded(const float x[5], const float *w, float ra[3], float rb[3]) {
     float a1[10], b1[10], a2[6], b2[6], a3[6], b3[6], za[3], zb[3];
     auto sw = [](float z) { return swish_folded(z); };
     auto id = [](float z) { return z; };
-    dense_pair<5, 10>(xa, xb, w, DEV_L1, a1, b1, sw);
+    dense_pair_reflected5(x, w, a1, b1, sw);
     dense_pair<10, 6>(a1, b1, w, DEV_L2, a2, b2, sw);
     dense_pair<6, 6>(a2, b2, w, DEV_L3, a3, b3, sw);
     dense_pair<6, 3>(a3, b3, w, DEV_L4, za, zb, id);
@@ -526,9 +548,8 @@ __device__ __forceinline__ double rcp_newton(double x) {
 // FP32 closure: reflection-averaged ratios from the FP32 features and folded parameters
 template <class MLP>
 __device__ __forceinline__ void sr_ratios32(const float f[5], const MLP &P, double br[3]) {
-    float xr[5] = {f[2], f[1], f[0], f[3], f[4]};
     float rd[3], rr[3];
-    mlp_pair32_folded(f, xr, P.p, rd, rr);
+    mlp_pair32_folded(f, P.p, rd, rr);
     br[0] = 0.5 * ((double)rd[0] + (double)rr[2]);
     br[1] = 0.5 * ((double)rd[1] + (double)rr[1]);
     br[2] = 0.5 * ((double)rd[2] + (double)rr[0]);
