@@ -263,6 +263,78 @@ int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double 
                                 void *workspace, size_t workspace_bytes, void *stream, double *sweep_ms,
                                 int32_t *launches);
 
+/* Ghost cells of u in place for cfg->boundary (4 ghost layers; double Mach at
+ * `time`): apply_boundary_kernel / apply_periodic_boundary_kernel of
+ * teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/warp_weno7_ader4_helpers_classical_only.py:2160-2200
+ * as point_rk4.py:453-455 launches them. */
+int wsr_euler7_boundary(const wsr_euler7_config *cfg, double *u, double time, void *stream);
+
+/* Max characteristic speed over the interior of u, returned to the host (synchronises
+ * the stream; NaN if any cell is NaN): the kernel and np.max of point_rk4.py:136-147
+ * compute_dt_from_warp_array.  The workspace needs WSR_SPEED_WORKSPACE_BYTES only (a
+ * wsr_euler7_workspace_bytes workspace serves as well). */
+#define WSR_SPEED_WORKSPACE_BYTES 4096u
+int wsr_euler7_max_speed(const wsr_euler7_config *cfg, const double *u, void *workspace, size_t workspace_bytes,
+                         double *max_speed_out, void *stream);
+
+/* ---- beta-provider protocol of the WENO7-SR point step (unfused path) ----
+ * The reference drives WENO7-SR through a beta provider: an object whose methods
+ * fill arrays of smoothness indicators (beta = 4 r of the reflection-averaged
+ * 6-24-16-16-4 closure, ones on a plateau), which separate kernels consume
+ * (teacherfree_lab_weno7_rk4_distance_balanced_fast/warp_sod/point_rk4_mlp.py:22-56,
+ * 194-273).  These entry points restore that interface; the fused wsr_euler7_* step
+ * is the production path and gives the same results for this library's closure.
+ * Arrays (FP64, C order, the reference's allocations):
+ *   beta_x (ny+8, nx+2, 32) / beta_y (ny+2, nx+8, 32): entry side*16 + comp*4 + k
+ *     (side 0: lr 1, the right face of the window's centre cell; side 1: lr 2);
+ *   beta_cross_x (ny, nx+2, 256) / beta_cross_y (ny+2, nx, 256): entry
+ *     group*128 + side*16 + comp*4 + k (group 0 from x_r / y_r, 1 from x_l / y_l;
+ *     side 0 serves Gauss location 1, side 1 location 2);
+ *   face values x_l, x_r, y_l, y_r and rhs arrays: padded (ny+8, nx+8, 4);
+ *   fluxes fx (ny, nx+1, 4), fy (ny+1, nx, 4).
+ * dir 1 = x, 2 = y.  The fill calls need cfg->scheme SR64 or SR32 and cfg->mlp, and
+ * a workspace of wsr_beta7_workspace_bytes() for the parameters. */
+size_t wsr_beta7_workspace_bytes(void);
+
+/* TorchWeno7PointBeta.fill_normal_x_for / fill_normal_y_for (point_rk4_mlp.py:24-37;
+ * for_paper_results/solvers/weno7_warp_beta.py:318-426, 616-648): the normal beta of
+ * every window of u (characteristic components, Roe state of each side's face). */
+int wsr_beta7_fill_normal(const wsr_euler7_config *cfg, const double *u, double *beta, int32_t dir, void *workspace,
+                          size_t workspace_bytes, void *stream);
+
+/* fill_cross_x_point / fill_cross_y_point (point_rk4_mlp.py:38-56;
+ * weno7_warp_beta.py:445-560, 650-682): the cross beta of the transverse windows of
+ * the face-value arrays (conservative components). */
+int wsr_beta7_fill_cross(const wsr_euler7_config *cfg, const double *right, const double *left, double *beta,
+                         int32_t dir, void *workspace, size_t workspace_bytes, void *stream);
+
+/* compute_x_stage_point_mlp_kernel / compute_y_stage_point_mlp_kernel
+ * (point_rk4_mlp.py:61-109 with warp_weno7_external_overwrite_fullall.py:84-211):
+ * both normal face values of every window from external beta (lr 1 -> right,
+ * lr 2 -> left). */
+int wsr_euler7_stage_ext(const wsr_euler7_config *cfg, const double *u, const double *beta, double *left,
+                         double *right, int32_t dir, void *stream);
+
+/* compute_x_flux_point_mlp_kernel / compute_y_flux_point_mlp_kernel
+ * (point_rk4_mlp.py:111-186): the cfg->riemann flux at Gauss location loca (1 or 2)
+ * of every face from external cross beta; reverse_upwind swaps the states; dt enters
+ * FORCE / EVILIN only. */
+int wsr_euler7_flux_ext(const wsr_euler7_config *cfg, const double *left, const double *right,
+                        const double *beta_cross, double *flux, int32_t dir, int32_t loca, double dt,
+                        int32_t reverse_upwind, void *stream);
+
+/* rhs_from_flux_kernel (point_rk4.py:290-312): interior cells of rhs from the four
+ * flux arrays (ghost entries are not written). */
+int wsr_euler7_rhs_from_flux(const wsr_euler7_config *cfg, double *rhs, const double *fx1, const double *fx2,
+                             const double *fy1, const double *fy2, void *stream);
+
+/* shu_stage1/2/3/final kernels (point_rk4.py:315-403) over every padded entry:
+ * stage 1 inputs (u, rhs0); 2 (u, rhs_t0, u1, rhs1); 3 (u, rhs_t0, u1, rhs_t1, u2,
+ * rhs2); 4 (u, rhs0, u1, rhs1, u2, u3, rhs3); out may alias an input only for stage 4
+ * and only the first (the reference writes the new state over u). */
+int wsr_euler7_shu_combine(const wsr_euler7_config *cfg, int32_t stage, double dt, const double *const *inputs,
+                           double *out, void *stream);
+
 /* ------------------------------------------------------------------ 3-D WENO5
  * The 3-D characteristic WENO5 step of code/snapshot/warp_weno5_3d_rk3:
  * state (nz+6, ny+6, nx+6, 5) FP64 conserved (rho, rho u, rho v, rho w, E),

@@ -83,3 +83,34 @@ def test_initial_state_argument_checks():
     assert L.wsr_init_vortex(a, 8, -1, 4, 0.0, -10.0, 10.0, -10.0, 10.0, 1.4, nodes, weights, 2, None) == -1
     lengths = (ctypes.c_double * 3)(1.0, 1.0, 1.0)
     assert L.wsr_init_shockbubble3d(a, 8, 8, 8, 3, 2, lengths, lengths, 0.1, 1.4, nodes, weights, 2, None) == -1
+
+
+def test_beta_protocol_argument_checks():
+    """The WENO7 beta-provider entry points refuse bad arguments before any launch."""
+    L = _lib.lib()
+    a = 1 << 12
+    cfg = _lib.Euler7Config(nx=8, ny=8, ghost=4, scheme=_lib.SCHEMES["js"], riemann=0, boundary=0, eno_cutoff=0,
+                            flags=0, loop_mode=0, reserved=0, dx=0.1, dy=0.1, gamma=1.4, cfl=0.4, t_end=1.0)
+    c = ctypes.byref(cfg)
+    assert L.wsr_beta7_workspace_bytes() >= 908 * 8
+    # the fill calls need an SR scheme and its parameters
+    assert L.wsr_beta7_fill_normal(c, a, a, 1, a, 1 << 20, None) == -1
+    cfg.scheme = _lib.SCHEMES["sr64"]
+    assert L.wsr_beta7_fill_normal(c, a, a, 1, a, 1 << 20, None) == -2
+    params = (ctypes.c_double * 908)()
+    cfg.mlp = ctypes.cast(params, ctypes.c_void_p)
+    assert L.wsr_beta7_fill_normal(c, a, a, 3, a, 1 << 20, None) == -1          # direction
+    assert L.wsr_beta7_fill_normal(c, a, a, 1, a, 16, None) == -3               # workspace
+    assert L.wsr_beta7_fill_cross(c, a, a, a + 8, 2, a, 1 << 20, None) == -1     # alignment
+    assert L.wsr_euler7_stage_ext(c, a, a, a, None, 1, None) == -1
+    assert L.wsr_euler7_flux_ext(c, a, a, a, a, 1, 3, 0.1, 0, None) == -1       # Gauss location
+    cfg.riemann = 9
+    assert L.wsr_euler7_flux_ext(c, a, a, a, a, 1, 1, 0.1, 0, None) == -1
+    cfg.riemann = 0
+    assert L.wsr_euler7_rhs_from_flux(c, a, a, a, None, a, None) == -1
+    inputs = (ctypes.c_void_p * 2)(a, a)
+    assert L.wsr_euler7_shu_combine(c, 5, 0.1, inputs, a, None) == -1
+    assert L.wsr_euler7_shu_combine(c, 2, 0.1, (ctypes.c_void_p * 4)(a, a, None, a), a, None) == -1
+    assert L.wsr_euler7_max_speed(c, a, a, 64, ctypes.byref(ctypes.c_double()), None) == -3
+    cfg.ghost = 3
+    assert L.wsr_euler7_boundary(c, a, 0.0, None) == -1

@@ -22,7 +22,9 @@ EXPORTS = (
     "wsr_euler2d_loop_steps_timed", "wsr_euler2d_counters", "wsr_adv1d_rollout",
     "wsr_adv1d_train_forward", "wsr_adv1d_train_adjoint",
     "wsr_euler7_workspace_bytes", "wsr_euler7_rhs", "wsr_euler7_step", "wsr_euler7_loop_init",
-    "wsr_euler7_loop_steps", "wsr_euler7_loop_steps_timed",
+    "wsr_euler7_loop_steps", "wsr_euler7_loop_steps_timed", "wsr_euler7_boundary", "wsr_euler7_max_speed",
+    "wsr_beta7_workspace_bytes", "wsr_beta7_fill_normal", "wsr_beta7_fill_cross", "wsr_euler7_stage_ext",
+    "wsr_euler7_flux_ext", "wsr_euler7_rhs_from_flux", "wsr_euler7_shu_combine",
     "wsr_euler3_workspace_bytes", "wsr_euler3_step", "wsr_euler3_max_speed", "wsr_euler3_loop_init",
     "wsr_euler3_loop_steps", "wsr_euler3_loop_steps_timed",
     "wsr_init_quadrant", "wsr_init_sod", "wsr_init_double_mach", "wsr_init_vortex", "wsr_init_shockbubble3d",
@@ -141,6 +143,24 @@ def lib() -> ctypes.CDLL:
         L.wsr_euler7_loop_steps.argtypes = [C7, P, P, I32, P, SZ, P]
         L.wsr_euler7_loop_steps_timed.restype = ctypes.c_int
         L.wsr_euler7_loop_steps_timed.argtypes = [C7, P, P, I32, P, SZ, P, ctypes.POINTER(D), ctypes.POINTER(I32)]
+        L.wsr_euler7_boundary.restype = ctypes.c_int
+        L.wsr_euler7_boundary.argtypes = [C7, P, D, P]
+        L.wsr_euler7_max_speed.restype = ctypes.c_int
+        L.wsr_euler7_max_speed.argtypes = [C7, P, P, SZ, ctypes.POINTER(D), P]
+        L.wsr_beta7_workspace_bytes.restype = SZ
+        L.wsr_beta7_workspace_bytes.argtypes = []
+        L.wsr_beta7_fill_normal.restype = ctypes.c_int
+        L.wsr_beta7_fill_normal.argtypes = [C7, P, P, I32, P, SZ, P]
+        L.wsr_beta7_fill_cross.restype = ctypes.c_int
+        L.wsr_beta7_fill_cross.argtypes = [C7, P, P, P, I32, P, SZ, P]
+        L.wsr_euler7_stage_ext.restype = ctypes.c_int
+        L.wsr_euler7_stage_ext.argtypes = [C7, P, P, P, P, I32, P]
+        L.wsr_euler7_flux_ext.restype = ctypes.c_int
+        L.wsr_euler7_flux_ext.argtypes = [C7, P, P, P, P, I32, I32, D, I32, P]
+        L.wsr_euler7_rhs_from_flux.restype = ctypes.c_int
+        L.wsr_euler7_rhs_from_flux.argtypes = [C7, P, P, P, P, P, P]
+        L.wsr_euler7_shu_combine.restype = ctypes.c_int
+        L.wsr_euler7_shu_combine.argtypes = [C7, I32, D, ctypes.POINTER(P), P, P]
         C3 = ctypes.POINTER(Euler3Config)
         for name, args in (("wsr_init_quadrant", [P, I32, I32, I32, I32, P, D, D, D, D, D, P, P, I32, P]),
                            ("wsr_init_sod", [P, I32, I32, I32, I32, D, D, D, P, P, I32, P]),

@@ -19,7 +19,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "libwsr_hip.so"
 OBJ_DIR = PKG / "_build"
 SOURCES = [CSRC / "wsr_euler2d.hip", CSRC / "wsr_euler7.hip", CSRC / "wsr_euler3.hip", CSRC / "wsr_adv1d.hip",
-           CSRC / "wsr_train1d.hip", CSRC / "wsr_init.hip"]
+           CSRC / "wsr_train1d.hip", CSRC / "wsr_init.hip", CSRC / "wsr_beta7.hip"]
 # the initial-state kernels reproduce the host builders' roundings: no contraction
 SOURCE_FLAGS = {"wsr_init.hip": ["-ffp-contract=off"]}
 HEADERS = [CSRC / "wsr_math.hpp", CSRC / "wsr_loop.hpp", CSRC / "wsr_weno7.hpp", CSRC / "wsr_adv1d.hpp",

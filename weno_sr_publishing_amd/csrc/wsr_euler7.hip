@@ -110,6 +110,46 @@ __global__ __launch_bounds__(BLOCK) void k7_boundary(const double *__restrict__ 
     d2[1] = make_double2(q[2], q[3]);
 }
 
+// The ghost cells of u in place (the reference's boundary kernels write the state they
+// are given): only ghost entries are written, and they read interior cells only.
+__global__ __launch_bounds__(BLOCK) void k7_ghosts_inplace(double *u, Grid g, int boundary, double time) {
+    int idx = blockIdx.x * BLOCK + threadIdx.x;
+    if (idx >= g.nxt * g.nyt) return;
+    int j = idx / g.nxt, i = idx - j * g.nxt;
+    if (i >= GC && i < g.nx + GC && j >= GC && j < g.ny + GC) return;
+    int si = i, sj = j;
+    if (boundary == WSR_BC_PERIODIC) {
+        if (i < GC) si = i + g.nx; else if (i >= g.nx + GC) si = i - g.nx;
+        if (j < GC) sj = j + g.ny; else if (j >= g.ny + GC) sj = j - g.ny;
+    } else {
+        if (i < GC) si = GC; else if (i >= g.nx + GC) si = g.nx + GC - 1;
+        if (j < GC) sj = GC; else if (j >= g.ny + GC) sj = g.ny + GC - 1;
+    }
+    const double *s = u + ((size_t)sj * g.nxt + si) * 4;
+    double q[4] = {s[0], s[1], s[2], s[3]};
+    if (boundary == WSR_BC_DOUBLE_MACH) {
+        double x = ((double)(i - GC) + 0.5) * g.dx;
+        double y = ((double)(j - GC) + 0.5) * g.dy;
+        const double x0 = 1.0 / 6.0;
+        if (j >= g.ny + GC) {
+            dm_exact(x, y, time, g.gamma, q);
+        } else if (j < GC) {
+            if (x < x0) {
+                dm_exact(x, 0.0, time, g.gamma, q);
+            } else {
+                int rj = 2 * GC - 1 - j;
+                int ri = i < GC ? GC : (i >= g.nx + GC ? g.nx + GC - 1 : i);
+                const double *r = u + ((size_t)rj * g.nxt + ri) * 4;
+                q[0] = r[0]; q[1] = r[1]; q[2] = -r[2]; q[3] = r[3];
+            }
+        } else if (i < GC) {
+            dm_exact(x, y, time, g.gamma, q);
+        }
+    }
+    double *d = u + (size_t)idx * 4;
+    d[0] = q[0]; d[1] = q[1]; d[2] = q[2]; d[3] = q[3];
+}
+
 // ------------------------------------------------------------------ sweep
 struct SweepArgs7 {
     Grid g;
@@ -493,6 +533,11 @@ __global__ void k7_zero_counters(Ctl *ctl) {
     if (threadIdx.x < NCOUNT) ctl->counters[threadIdx.x] = 0ull;
 }
 
+__global__ void k7_reset_speed(Ctl *ctl) {
+    ctl->speed_bits = 0ull;
+    ctl->nan_flag = 0;
+}
+
 __global__ void k7_loop_init(Ctl *ctl, double t_end, double cfl_h, int max_steps) {
     ctl->t = 0.0; ctl->dt = 0.0; ctl->t_end = t_end; ctl->cfl_h = cfl_h; ctl->dt_last = 0.0;
     ctl->speed_bits = 0ull; ctl->step = 0; ctl->done = 0; ctl->nan_flag = 0; ctl->max_steps = max_steps;
@@ -826,6 +871,41 @@ int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double 
     for (int i = 0; i < 2 * tm->cap; ++i) (void)hipEventDestroy(tm->ev[i]);
     delete tm;
     return rc;
+}
+
+int wsr_euler7_boundary(const wsr_euler7_config *cfg, double *u, double time, void *stream) {
+    int rc = check7(cfg);
+    if (rc) return rc;
+    if (!u) return WSR_E_ARG;
+    Grid g = grid7(cfg);
+    const int np = g.nxt * g.nyt;
+    hipLaunchKernelGGL(k7_ghosts_inplace, dim3((np + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, (hipStream_t)stream, u, g,
+                       cfg->boundary, time);
+    return hipGetLastError() == hipSuccess ? WSR_OK : WSR_E_HIP;
+}
+
+int wsr_euler7_max_speed(const wsr_euler7_config *cfg, const double *u, void *workspace, size_t workspace_bytes,
+                         double *max_speed_out, void *stream) {
+    int rc = check7(cfg);
+    if (rc) return rc;
+    if (!u || !workspace || !max_speed_out) return WSR_E_ARG;
+    if (!all_aligned16(u, workspace)) return WSR_E_ARG;
+    // the reduction uses the control block at the start of the workspace only
+    static_assert(sizeof(Ctl) <= WSR_SPEED_WORKSPACE_BYTES, "control block fits the speed workspace");
+    if (workspace_bytes < WSR_SPEED_WORKSPACE_BYTES) return WSR_E_WORKSPACE;
+    Ctl *ctl = (Ctl *)workspace;
+    hipStream_t st = (hipStream_t)stream;
+    Grid g = grid7(cfg);
+    hipLaunchKernelGGL(k7_reset_speed, dim3(1), dim3(1), 0, st, ctl);
+    const int nc = g.nx * g.ny;
+    hipLaunchKernelGGL(k7_max_speed, dim3((nc + BLOCK - 1) / BLOCK), dim3(BLOCK), 0, st, u, g, ctl);
+    Ctl host;
+    if (hipMemcpyAsync(&host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, st) != hipSuccess) return WSR_E_HIP;
+    if (hipStreamSynchronize(st) != hipSuccess) return WSR_E_HIP;
+    double m;
+    memcpy(&m, &host.speed_bits, sizeof m);
+    *max_speed_out = host.nan_flag ? NAN : m;
+    return WSR_OK;
 }
 
 }  // extern "C"

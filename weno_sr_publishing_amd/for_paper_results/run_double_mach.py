@@ -136,6 +136,9 @@ def main() -> None:
     parser.add_argument("--init-quadrature", type=int, default=15)
     parser.add_argument("--device", default="cuda")
     parser.add_argument("--report-interval", type=int, default=100)
+    # run_double_mach.py:63-68: both providers evaluate the same FP64 checkpoint (here the
+    # same HIP closure), so the choice does not change the run
+    parser.add_argument("--weno7-beta-backend", choices=("warp", "torch"), default="warp")
     parser.add_argument("--methods", default="weno5_sr_f64,weno5_sr_f32,weno7_js,weno7_sr_f64")
     parser.add_argument("--z7-unit-gauss-h", action="store_true")
     args = parser.parse_args()

@@ -14,6 +14,7 @@ from typing import Callable
 
 import numpy as np
 
+from .. import beta7
 from ..euler7 import Euler7Solver, Params7 as Params, double_mach_params7
 from .point_rk4_mlp import TorchWeno7PointBeta
 
@@ -24,6 +25,12 @@ def run_to_time(initial: np.ndarray, params: Params, t_end: float, device: str,
                 beta_model: TorchWeno7PointBeta | None, *, report_interval: int = 100,
                 report: Callable[[int, float, float, dict], None] | None = None,
                 riemann_solver: str = "hllc") -> tuple[np.ndarray, list[float], int, float]:
+    """WENO7-JS (beta_model None) or WENO7-SR; this library's provider runs on the fused
+    step, any other beta provider through the reference's protocol (weno7_double_mach.py:
+    275-453 compute_rhs_mlp / _launch_shu_rk4_step on the HIP consumer kernels)."""
+    if beta_model is not None and not beta7.is_fusable(beta_model):
+        return beta7.run_double_mach(initial, params, t_end, device, beta_model, report_interval=report_interval,
+                                     report=report, riemann=riemann_solver)
     scheme = "js" if beta_model is None else beta_model.scheme
     mlp = None if beta_model is None else beta_model.mlp
     solver = Euler7Solver(params, scheme, riemann_solver, "double-mach", mlp, loop="double-mach", device=device)
