@@ -380,6 +380,20 @@ def sweep7_flops_per_launch(scheme: str, frac_n: float, frac_g: float, both: boo
     return total
 
 
+# the multiply-adds of one 6-24-16-16-4 pass (6x24 + 24x16 + 16x16 + 16x4, 2 ops each): in
+# the fused FP32 sweep they run on the matrix cores, the rest of the pass on the VALU
+MLP7_MADD_OPS = 2 * (6 * 24 + 24 * 16 + 16 * 16 + 16 * 4)
+
+
+def sweep7_mfma_ops_per_launch(frac_n: float, frac_g: float) -> float:
+    """Closure multiply-add ops of one k7_sweep_xy launch at 1200x300 (both passes of every
+    closure evaluation; the closure fractions as in sweep7_flops_per_launch)."""
+    total = 0.0
+    for nfaces, ntrans in ((NX7 + 1, NY7), (NY7 + 1, NX7)):
+        total += nfaces * (ntrans + 6) * 8 * frac_n + nfaces * ntrans * 8 * frac_g
+    return total * 2 * MLP7_MADD_OPS
+
+
 def bench_doublemach7(args) -> int:
     import torch
     from weno_sr_publishing_amd.checkpoints import WENO7_SR_FP64, load_weno7_checkpoint
@@ -482,8 +496,13 @@ def bench_doublemach7(args) -> int:
         "sr_vs_js_wall": wall_sr / wall_js,
         "roofline": {
             "bound": "valu",
-            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
-                       "FP64-vector / non-packed FP32-vector rate",
+            "compute": ("VALU (FP64 reconstruction, Riemann flux, closure activations) + MFMA "
+                        "v_mfma_f32_4x4x1f32 (the FP32 closure's multiply-adds; share of the algorithmic ops "
+                        "in mfma_ops_share); peak is the MI355X FP64-vector / non-packed FP32-vector rate"
+                        if closure == "sr32" else
+                        "VALU (no MFMA): FP64 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
+                        "FP64-vector / non-packed FP32-vector rate"),
+            "mfma_ops_share": (sweep7_mfma_ops_per_launch(frac_n, frac_g) / flops if closure == "sr32" else 0.0),
             "kernel": f"k7_sweep_xy<{closure},hllc,L+L~> (x and y tiles of a right-hand side in one launch)",
             "achieved": achieved,
             "peak": PEAK_VALU_TFLOPS,

@@ -21,7 +21,9 @@ OBJ_DIR = PKG / "_build"
 SOURCES = [CSRC / "wsr_euler2d.hip", CSRC / "wsr_euler7.hip", CSRC / "wsr_euler3.hip", CSRC / "wsr_adv1d.hip",
            CSRC / "wsr_train1d.hip", CSRC / "wsr_init.hip", CSRC / "wsr_beta7.hip"]
 # the initial-state kernels reproduce the host builders' roundings: no contraction
-SOURCE_FLAGS = {"wsr_init.hip": ["-ffp-contract=off"]}
+SOURCE_FLAGS = {"wsr_init.hip": ["-ffp-contract=off"],
+                # the FP32 WENO7 closure's matrix-core results stay in VGPRs (no AGPR reads)
+                "wsr_euler7.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 HEADERS = [CSRC / "wsr_math.hpp", CSRC / "wsr_loop.hpp", CSRC / "wsr_weno7.hpp", CSRC / "wsr_adv1d.hpp",
            CSRC / "wsr_adjoint.hpp", PKG.parent / "include" / "wsr.h"]
 ARCH = os.environ.get("WSR_OFFLOAD_ARCH", "gfx950")

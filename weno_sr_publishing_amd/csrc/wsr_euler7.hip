@@ -184,7 +184,7 @@ template <int DIR, int SCHEME, int RIEMANN, bool BOTH>
 __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, double *__restrict__ F,
                                             double *__restrict__ Ft, const SweepArgs7 &A,
                                             const typename Mlp7Sweep<SCHEME>::type &mlp, double dt, double *smem,
-                                            int bx, int by) {
+                                            float *sA, int bx, int by) {
     const Grid &g = A.g;
     const double h_n = DIR == 1 ? g.dx : g.dy;
     const double h_t = A.unit_gauss_h ? 1.0 : (DIR == 1 ? g.dy : g.dx);
@@ -213,7 +213,11 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
         double *d = T + (n * NT1 + tau) * 4;
         d[0] = a.x; d[1] = a.y; d[2] = b.x; d[3] = b.y;
     }
+    // the FP32 closure's matrix-core weights (A registers) into LDS
+    if constexpr (SCHEME == SCHEME_SR32)
+        for (int e = threadIdx.x; e < W7M_WORDS; e += SBLOCK) sA[e] = mlp.p[e];
     __syncthreads();
+    const Mfma7A wa{sA, (int)(threadIdx.x & 63)};
 
     // phase 1a: Roe average of the two cells next to each face (cells n = fl+3, fl+4)
     if (threadIdx.x < N1) {
@@ -250,8 +254,10 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
             c[m] = (l[0] * q[0] + l[1] * q[1] + l[2] * q[2] + l[3] * q[3]) * scale;
         }
         bool closure, trip;
-        double val = s == 0 ? recon7_normal<SCHEME, 1>(c, mlp, A.eno, h_n, zf, closure, trip)
-                            : recon7_normal<SCHEME, 2>(c, mlp, A.eno, h_n, zf, closure, trip);
+        double val;
+        if constexpr (SCHEME == SCHEME_SR32) val = recon7_normal_mfma(c, s, wa, A.eno, h_n, zf, closure, trip);
+        else val = s == 0 ? recon7_normal<SCHEME, 1>(c, mlp, A.eno, h_n, zf, closure, trip)
+                          : recon7_normal<SCHEME, 2>(c, mlp, A.eno, h_n, zf, closure, trip);
         if (A.counters) cnt.add(0, 4, true, closure);
         if (zf) {
             // the four lanes of one side hold the four characteristic components of one state
@@ -282,17 +288,22 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
             const int ci = base + slot;
             const int it = c0 + ci;
             // items past the tile's N2 faces (whole waves of the last pass at 4 x 26)
-            // have no face: no reconstruction, no flux
-            if (it >= N2) continue;
+            // have no face: no reconstruction, no flux.  The skip is decided per wave (a
+            // uniform branch keeps EXEC full for the matrix-core closure); a wave with some
+            // items past the faces (none at 4 x 26) reconstructs a valid item in their lanes
+            // and drops the results.
+            const bool valid = it < N2;
+            if (__ballot(valid) == 0ull) continue;
             int fl, tl;
-            item_map<DIR>(it, TT, fl, tl);
+            item_map<DIR>(valid ? it : N2 - 1, TT, fl, tl);
             double q[7];
 #pragma unroll
             for (int m = 0; m < 7; ++m) q[m] = FV[((tl + m) * TN + fl) * 8 + gi];
             double g1, g2;
             bool closure, trip1, trip2;
-            recon7_gauss<SCHEME>(q, mlp, A.eno, h_t, zf, g1, g2, closure, trip1, trip2);
-            if (A.counters) cnt.add(2, 5, (f0 + fl) < nfaces && (t0 + tl) < ntrans, closure);
+            if constexpr (SCHEME == SCHEME_SR32) recon7_gauss_mfma(q, wa, A.eno, h_t, zf, g1, g2, closure, trip1, trip2);
+            else recon7_gauss<SCHEME>(q, mlp, A.eno, h_t, zf, g1, g2, closure, trip1, trip2);
+            if (A.counters) cnt.add(2, 5, valid && (f0 + fl) < nfaces && (t0 + tl) < ntrans, valid && closure);
             if (zf) {
                 // lanes of a group of four hold one side's four conservative components
                 // (items past the tile skip whole groups of eight lanes)
@@ -301,10 +312,13 @@ __device__ __forceinline__ void sweep7_tile(const double *__restrict__ bc, doubl
                 if (z1) g1 = z7_value<4>(q, h_t);
                 if (z2) g2 = z7_value<3>(q, h_t);
                 // states of faces inside the grid (the tile's edge lines past it are not counted)
-                if (k == 0 && (f0 + fl) < nfaces && (t0 + tl) < ntrans) nfell += (z1 ? 1u : 0u) + (z2 ? 1u : 0u);
+                if (valid && k == 0 && (f0 + fl) < nfaces && (t0 + tl) < ntrans)
+                    nfell += (z1 ? 1u : 0u) + (z2 ? 1u : 0u);
             }
-            GV[ci * 16 + gi] = g1;
-            GV[ci * 16 + 8 + gi] = g2;
+            if (valid) {
+                GV[ci * 16 + gi] = g1;
+                GV[ci * 16 + 8 + gi] = g2;
+            }
         }
         __syncthreads();
         if (c0 + (int)(threadIdx.x >> 1) < N2) {
@@ -359,14 +373,15 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(Sweep7Wa
         dt = A.ctl->dt;
     }
     __shared__ double smem[FV_SIZE + REG_SIZE];
+    __shared__ float sA[SCHEME == SCHEME_SR32 ? W7M_WORDS : 1];
     const int gx = (A.g.nx + 1 + TN - 1) / TN, tx = (A.g.ny + TT - 1) / TT;
     const int gy = (A.g.ny + 1 + TN - 1) / TN;
     const int id = blockIdx.x;
     if (id < gx * tx) {
-        sweep7_tile<1, SCHEME, RIEMANN, BOTH>(bc, Fx, Fxt, A, mlp, dt, smem, id % gx, id / gx);
+        sweep7_tile<1, SCHEME, RIEMANN, BOTH>(bc, Fx, Fxt, A, mlp, dt, smem, sA, id % gx, id / gx);
     } else {
         const int iy = id - gx * tx;
-        sweep7_tile<2, SCHEME, RIEMANN, BOTH>(bc, Fy, Fyt, A, mlp, dt, smem, iy % gy, iy / gy);
+        sweep7_tile<2, SCHEME, RIEMANN, BOTH>(bc, Fy, Fyt, A, mlp, dt, smem, sA, iy % gy, iy / gy);
     }
 }
 
@@ -596,7 +611,9 @@ static size_t ws7_layout(int nx, int ny, Workspace7 *w, void *base) {
     char *b = (char *)base;
     auto take = [&](size_t bytes) { char *p = b + off; off += align256(bytes); return (void *)p; };
     void *ctl = take(sizeof(Ctl));
-    void *mlp = take(MLP7_COUNT * sizeof(double));
+    // FP64 closure: the neuron-major block; FP32: the matrix-core A table
+    void *mlp = take(MLP7_COUNT * sizeof(double) > W7M_WORDS * sizeof(float) ? MLP7_COUNT * sizeof(double)
+                                                                             : W7M_WORDS * sizeof(float));
     void *bc = take(np), *fx = take(nfx), *fy = take(nfy), *fxt = take(nfx), *fyt = take(nfy);
     void *u1 = take(np), *u2 = take(np), *u3 = take(np), *rt0 = take(np), *acc = take(np);
     if (w && base) {
@@ -633,7 +650,8 @@ static Grid grid7(const wsr_euler7_config *cfg) {
 
 static void enqueue_params7(const wsr_euler7_config *cfg, const Workspace7 &w, hipStream_t st) {
     if (cfg->scheme != WSR_SCHEME_SR64 && cfg->scheme != WSR_SCHEME_SR32) return;
-    uint32_t words[MLP7_COUNT * 2];
+    static_assert(W7M_WORDS <= MLP7_COUNT * 2 + 64, "the word buffer holds either parameter block");
+    uint32_t words[MLP7_COUNT * 2 + 64];
     int nwords;
     if (cfg->scheme == WSR_SCHEME_SR64) {
         double dev[MLP7_COUNT];
@@ -641,10 +659,13 @@ static void enqueue_params7(const wsr_euler7_config *cfg, const Workspace7 &w, h
         memcpy(words, dev, sizeof dev);
         nwords = MLP7_COUNT * 2;
     } else {
-        float dev[MLP7_COUNT];
+        // the FP32 closure runs on the matrix cores: folded parameters as the A table
+        float dev[MLP7_COUNT], table[W7M_WORDS];
         pack7_neuron_major((const float *)cfg->mlp, dev);
-        memcpy(words, dev, sizeof dev);
-        nwords = MLP7_COUNT;
+        fold7_fp32(dev);
+        mfma7_table(dev, table);
+        memcpy(words, table, sizeof table);
+        nwords = W7M_WORDS;
     }
     for (int o = 0; o < nwords; o += PCHUNK) {
         ParamChunk c;

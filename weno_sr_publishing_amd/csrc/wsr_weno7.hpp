@@ -259,6 +259,174 @@ __device__ __forceinline__ bool closure_beta7(const double v[7], bool reversed,
     return true;
 }
 
+// ------------------------------------ the FP32 closure on the matrix cores
+// The fused sweep evaluates the FP32 6-24-16-16-4 closure of a whole wave with
+// v_mfma_f32_4x4x1f32 (16 blocks of 4 lanes): the B operand is each lane's own input
+// (so a lane's stencil stays in its lane), the A operand four neurons' weights, read from
+// one register for 16 instructions through the A broadcast (CBSZ 4, ABID = entry mod 16),
+// and the 4 x 4 result of block b leaves neuron i of lane 4b + j's stencil in register i
+// of lane 4b + j.  Every layer is laid out as groups of four neurons; the direct and the
+// reflected evaluation are separate groups (reflected layer-1 weights permuted, so both
+// read the direct features), and each group sums its bias first (B = 1) and then its
+// inputs in ascending order, the order of the VALU closure.  Hidden activations and the
+// head use the folded constants of fold7_fp32 (the swish z' / (1 + 2^z'), the bounded
+// head 2^(6L - 12L / (1 + 2^u))).  Measured on MI355X against the VALU pair: 1.48x
+// (9.6k against 14.2k SIMD cycles per wave).
+typedef float v4f32 __attribute__((ext_vector_type(4)));
+constexpr int W7M_L1 = 0, W7M_L2 = 84, W7M_L3 = 284, W7M_L4 = 420;
+constexpr int W7M_ENTRIES = 454;                       // A entries (4 floats each)
+constexpr int W7M_VGPRS = (W7M_ENTRIES + 15) / 16;     // 16 entries per A register
+constexpr int W7M_WORDS = W7M_VGPRS * 64;              // table floats: register v, lane l at v * 64 + l
+
+// host: the activation constants folded into the neuron-major FP32 block (as
+// fold_fp32_closure for the 5-10-6-6-3 closure): layer-1 weights and all hidden biases
+// scaled by -log2(e), output weights by -1/3 and output biases by log2(e)/3
+inline void fold7_fp32(float *dev) {
+    const int start[4] = {D7_L1, D7_L2, D7_L3, D7_L4}, K[4] = {6, 24, 16, 16}, N[4] = {24, 16, 16, 4};
+    for (int l = 0; l < 4; ++l)
+        for (int n = 0; n < N[l]; ++n) {
+            float *run = dev + start[l] + n * (K[l] + 1);
+            const double wscale = l == 0 ? -LOG2E : (l == 3 ? -1.0 / 3.0 : 1.0);
+            const double bscale = l == 3 ? LOG2E / 3.0 : -LOG2E;
+            run[0] = (float)((double)run[0] * bscale);
+            for (int k = 1; k <= K[l]; ++k) run[k] = (float)((double)run[k] * wscale);
+        }
+}
+
+// host: the A table from the folded neuron-major block.  Entry m of four floats (neuron
+// p of its group) sits at T[m * 4 + p]; register v of lane l holds T[v * 64 + l].
+// Layer 1: groups 0..5 direct neurons, 6..11 reflected (input k reads feature P6(k));
+// layers 2, 3: groups 0..3 direct, 4..7 reflected; layer 4: group 0 direct, 1 reflected.
+inline void mfma7_table(const float *dev, float *T) {
+    for (int i = 0; i < W7M_WORDS; ++i) T[i] = 0.0f;
+    const int reflect[6] = {3, 2, 1, 0, 4, 5};
+    for (int G = 0; G < 12; ++G)
+        for (int p = 0; p < 4; ++p) {
+            const int n = 4 * G + p, o = n % 24;
+            const bool refl = n >= 24;
+            T[(W7M_L1 + G * 7) * 4 + p] = dev[D7_L1 + o * 7];
+            for (int k = 0; k < 6; ++k)
+                T[(W7M_L1 + G * 7 + 1 + k) * 4 + p] = dev[D7_L1 + o * 7 + 1 + (refl ? reflect[k] : k)];
+        }
+    auto hidden = [&](int m0, int start, int K) {
+        for (int G = 0; G < 8; ++G)
+            for (int p = 0; p < 4; ++p) {
+                const int o = (4 * G + p) % 16;
+                T[(m0 + G * (K + 1)) * 4 + p] = dev[start + o * (K + 1)];
+                for (int k = 0; k < K; ++k) T[(m0 + G * (K + 1) + 1 + k) * 4 + p] = dev[start + o * (K + 1) + 1 + k];
+            }
+    };
+    hidden(W7M_L2, D7_L2, 24);
+    hidden(W7M_L3, D7_L3, 16);
+    for (int G = 0; G < 2; ++G)
+        for (int p = 0; p < 4; ++p) {
+            T[(W7M_L4 + G * 17) * 4 + p] = dev[D7_L4 + p * 17];
+            for (int k = 0; k < 16; ++k) T[(W7M_L4 + G * 17 + 1 + k) * 4 + p] = dev[D7_L4 + p * 17 + 1 + k];
+        }
+}
+
+// A registers of the closure: the table in LDS, register v of this lane
+struct Mfma7A {
+    const float *lds;
+    int lane;
+    __device__ __forceinline__ float operator()(int v) const { return lds[v * 64 + lane]; }
+};
+
+template <int M>
+__device__ __forceinline__ v4f32 mfma7(float a, float b, v4f32 c) {
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 4, M & 15, 0);
+}
+// one input step I of every group (independent accumulators back to back)
+template <int M0, int K, int NG, bool SPLIT, int I, int G>
+__device__ __forceinline__ void mfma7_kstep(const Mfma7A &wa, const float *x, v4f32 *acc) {
+    if constexpr (G < NG) {
+        constexpr int M = M0 + G * (K + 1) + 1 + I;
+        acc[G] = mfma7<M>(wa(M >> 4), x[(SPLIT && G >= NG / 2 ? K : 0) + I], acc[G]);
+        mfma7_kstep<M0, K, NG, SPLIT, I, G + 1>(wa, x, acc);
+    }
+}
+template <int M0, int K, int NG, bool SPLIT, int I>
+__device__ __forceinline__ void mfma7_kloop(const Mfma7A &wa, const float *x, v4f32 *acc) {
+    if constexpr (I < K) {
+        mfma7_kstep<M0, K, NG, SPLIT, I, 0>(wa, x, acc);
+        mfma7_kloop<M0, K, NG, SPLIT, I + 1>(wa, x, acc);
+    }
+}
+template <int M0, int K, int NG, int G>
+__device__ __forceinline__ void mfma7_bias(const Mfma7A &wa, v4f32 *acc) {
+    if constexpr (G < NG) {
+        constexpr int M = M0 + G * (K + 1);
+        const v4f32 z = {0.0f, 0.0f, 0.0f, 0.0f};
+        acc[G] = mfma7<M>(wa(M >> 4), 1.0f, z);
+        mfma7_bias<M0, K, NG, G + 1>(wa, acc);
+    }
+}
+// NG groups of four neurons over K inputs (SPLIT: the second half of the groups reads
+// inputs K..2K-1), swish on the hidden layers
+template <int M0, int K, int NG, bool SPLIT, bool ACT>
+__device__ __forceinline__ void mfma7_layer(const Mfma7A &wa, const float *x, float *h) {
+    v4f32 acc[NG];
+    mfma7_bias<M0, K, NG, 0>(wa, acc);
+    mfma7_kloop<M0, K, NG, SPLIT, 0>(wa, x, acc);
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) h[4 * g + p] = ACT ? swish_folded(acc[g][p]) : acc[g][p];
+}
+
+__device__ __forceinline__ void head4_folded(const float u[4], float r[4]) {
+    const float c6 = (float)(6.0 * LOG2E), c12 = (float)(12.0 * LOG2E);
+    float e[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        float q = rcp_fast(1.0f + __builtin_amdgcn_exp2f(u[k]));
+        e[k] = __builtin_amdgcn_exp2f(fmaf(-c12, q, c6));
+    }
+    float inv = rcp_fast((e[0] + e[1]) + (e[2] + e[3]));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r[k] = e[k] * inv;
+}
+
+// ra = M(x), rb = M(P6 x) for every lane of the wave (call with all 64 lanes active)
+__device__ __forceinline__ void closure7_mfma(const float x[6], const Mfma7A &wa, float ra[4], float rb[4]) {
+    float h1[48], h2[32], h3[32], z4[8];
+    mfma7_layer<W7M_L1, 6, 12, false, true>(wa, x, h1);
+    mfma7_layer<W7M_L2, 24, 8, true, true>(wa, h1, h2);
+    mfma7_layer<W7M_L3, 16, 8, true, true>(wa, h2, h3);
+    mfma7_layer<W7M_L4, 16, 2, true, false>(wa, h3, z4);
+    head4_folded(z4, ra);
+    head4_folded(z4 + 4, rb);
+}
+
+// closure_beta7 for the FP32 closure of a whole wave (every lane calls it with all 64
+// lanes active; lanes on a plateau get ones).  The matrix cores run when any lane of the
+// wave needs the closure.
+__device__ __forceinline__ bool closure_beta7_wave(const double v[7], bool reversed, const Mfma7A &wa,
+                                                   double beta[4], bool &invalid) {
+    double q[7], f[6];
+#pragma unroll
+    for (int m = 0; m < 7; ++m) q[m] = reversed ? v[6 - m] : v[m];
+    const bool active = features7(q, f);
+    float x[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[k] = active ? (float)f[k] : 0.0f;
+    double r[4] = {0.0, 0.0, 0.0, 0.0};
+    if (__ballot(active) != 0ull) {   // wave-uniform branch: EXEC stays full
+        float ra[4], rb[4];
+        closure7_mfma(x, wa, ra, rb);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = 0.5 * ((double)ra[k] + (double)rb[3 - k]);
+    }
+    invalid = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double rk = reversed ? r[3 - k] : r[k];
+        invalid = invalid || (active && !(rk >= 0.0 && rk <= 1.0));
+        beta[k] = active ? 4.0 * rk : 1.0;
+    }
+    return active;
+}
+
 // EXT:64-79 weights from beta: a = d / (beta + 1e-12)^2, normalised (+ ENO cutoff)
 __device__ __forceinline__ double combine_beta7(const double s[4], const double d[4], const double beta[4], int eno) {
     double a[4];
@@ -370,6 +538,55 @@ __device__ __forceinline__ double recon7_normal(const double q[7], const typenam
         }
         return combine_beta7(s, d, b, eno);
     }
+}
+
+// recon7_normal for the FP32 closure on the matrix cores: side s of the lane (0: lr 1,
+// head 1; 1: lr 2, head 2, the mirrored stencil) chosen by selects, so that every lane
+// reaches the closure together.
+__device__ __forceinline__ double recon7_normal_mfma(const double q[7], int s, const Mfma7A &wa, int eno, double h,
+                                                     bool z_fallback, bool &closure, bool &trip) {
+    double s1[4], s2[4], d1[4], d2[4], sv[4], d[4], b[4];
+    candidates7<1>(q, s1);
+    candidates7<2>(q, s2);
+    optimal7<1>(d1);
+    optimal7<2>(d2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        sv[k] = s ? s2[k] : s1[k];
+        d[k] = s ? d2[k] : d1[k];
+    }
+    bool invalid;
+    closure = closure_beta7_wave(q, s == 1, wa, b, invalid);
+    trip = false;
+    if (z_fallback && closure) {
+        double bz[4];
+        beta7_sos(q, bz);
+        trip = z7_guard_trips(d, b, bz, h, invalid);
+    }
+    return combine_beta7(sv, d, b, eno);
+}
+
+// recon7_gauss for the FP32 closure on the matrix cores (every lane of the wave calls it)
+__device__ __forceinline__ void recon7_gauss_mfma(const double q[7], const Mfma7A &wa, int eno, double h,
+                                                  bool z_fallback, double &g1, double &g2, bool &closure,
+                                                  bool &trip1, bool &trip2) {
+    double s1[4], s2[4], d1[4], d2[4], b[4];
+    candidates7<4>(q, s1);
+    candidates7<3>(q, s2);
+    optimal7<4>(d1);
+    optimal7<3>(d2);
+    bool invalid;
+    closure = closure_beta7_wave(q, false, wa, b, invalid);
+    trip1 = false;
+    trip2 = false;
+    if (z_fallback && closure) {
+        double bz[4];
+        beta7_sos(q, bz);
+        trip1 = z7_guard_trips(d1, b, bz, h, invalid);
+        trip2 = z7_guard_trips(d2, b, bz, h, invalid);
+    }
+    g1 = combine_beta7(s1, d1, b, eno);
+    g2 = combine_beta7(s2, d2, b, eno);
 }
 
 // Both transverse Gauss-point values (loca 1 = head 4, loca 2 = head 3) of one
