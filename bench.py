@@ -91,7 +91,7 @@ SWEEP_TRAFFIC_BYTES_400 = 38.06e6
 # the same for the SR32 3-D flux launch at 224x88x88 (mean of the x, y, z face
 # passes; profiles/pmc_traffic_k3_shockbubble_224x88x88.md) and the SR32 L+L~ WENO7
 # sweep launch at 1200x300 (x and y tiles; profiles/pmc_traffic_k7_sweep_xy_doublemach_1200x300.md)
-FLUX3_TRAFFIC_BYTES_224 = 841.0e6
+FLUX3_TRAFFIC_BYTES_224 = 1208.6e6
 SWEEP7_TRAFFIC_BYTES_1200 = 135.3e6
 
 

@@ -83,15 +83,16 @@ __device__ __forceinline__ long long axis_stride(const Grid3 &g) {
 // y then z, y faces x then z, z faces y then x)
 template <int DIR> struct Trans { static constexpr int T1 = DIR == 2 ? 1 : 2, T2 = DIR == 3 ? 1 : 3; };
 
-// Work item of a pass over a box ex x ey x ez (x fastest).  A pass whose stencil runs
-// along y or z (SAX = 2 / 3) gives each workgroup a tile of 16 x positions times NPB / 16
-// lines along SAX, so that the 5-point stencils of one workgroup overlap in the same
-// cache lines; in plain x-row order the neighbouring stencil lines belong to workgroups
-// dispatched to other XCDs, whose L2s each fetch them (4-5x the algorithmic reads).
+// Work item of a pass over a box ex x ey x ez (x fastest).  SAX = 1 walks x rows.
+// SAX = 2 / 3 gives each workgroup a tile of TILE_X x positions times NPB / TILE_X lines
+// along y / z, so that the 5-point stencils of one workgroup (along that axis) overlap
+// in the same cache lines; in plain x-row order the neighbouring stencil lines belong
+// to workgroups dispatched to other XCDs, whose L2s each fetch them (4-5x the
+// algorithmic reads).  A wave of a tile is also a compact patch of the grid, which
+// matters where a closure runs: fewer waves straddle a shock or an interface.
 // active is false for the items of a partial tile (their coordinates are clamped
 // into the box, so that their loads stay in bounds).
-constexpr int TILE_X = 16;
-template <int SAX, int NPB>
+template <int SAX, int NPB, int TILE_X = 16>
 __device__ __forceinline__ void tile_item(int f, int ex, int ey, int ez, int &i, int &j, int &k, bool &active) {
     if constexpr (SAX == 1) {
         const long long item = (long long)blockIdx.x * NPB + f;
@@ -102,6 +103,7 @@ __device__ __forceinline__ void tile_item(int f, int ex, int ey, int ez, int &i,
         j = (int)(r % ey);
         k = (int)(r / ey);
     } else {
+        static_assert(NPB % TILE_X == 0, "a tile holds whole x rows");
         constexpr int TS = NPB / TILE_X;
         const int es = SAX == 2 ? ey : ez;
         const int nbx = (ex + TILE_X - 1) / TILE_X, nbs = (es + TS - 1) / TS;
@@ -117,14 +119,20 @@ __device__ __forceinline__ void tile_item(int f, int ex, int ey, int ez, int &i,
         else { k = sc; j = other; }
     }
 }
-// The normal and first transverse passes tile for the closures (measured: SR32 normal
-// y / z 325 -> 245 us, first pass along y 226 / 213 -> 163 / 165 us); WENO-JS, whose light
-// reconstructions leave these passes waiting on memory, keeps the x-row order there (its
-// tiled passes measured 8 % slower).
-template <int SCHEME, int SAX> struct TileAxis { static constexpr int value = SCHEME == SCHEME_JS ? 1 : SAX; };
+// Tile axis and width per pass (PASS 0 normal, 1 first transverse, 2 flux).  The
+// closure passes tile along the stencil axis, or along y when the stencil runs along
+// x, 8 wide for the normal and flux passes (8 x 8 / 8 x 2 patches per wave) and 16 wide
+// for the first transverse pass.  Measured on the 224 x 88 x 88 shock-bubble (SR32):
+// x rows everywhere 8.78 ms/step, 16-wide tiles along y / z 8.03, also for the x
+// passes 7.55, these widths 7.28; 4-wide tiles were slower.  WENO-JS, whose light
+// reconstructions leave these passes waiting on memory, keeps x rows for its normal
+// and first passes (tiled: 8 % slower) and 16-wide tiles for its flux.
+template <int SCHEME, int SAX> struct TileAxis { static constexpr int value = SCHEME == SCHEME_JS ? 1 : (SAX == 1 ? 2 : SAX); };
+template <int SCHEME, int SAX> struct FluxAxis { static constexpr int value = SCHEME == SCHEME_JS ? SAX : (SAX == 1 ? 2 : SAX); };
+template <int SCHEME, int PASS> struct TileWidth { static constexpr int value = SCHEME == SCHEME_JS ? 16 : (PASS == 1 ? 16 : 8); };
 
 // workgroups of a tile_item pass
-template <int SAX, int NPB>
+template <int SAX, int NPB, int TILE_X = 16>
 static inline unsigned tile_blocks(int ex, int ey, int ez) {
     if (SAX == 1) return (unsigned)(((long long)ex * ey * ez + NPB - 1) / NPB);
     constexpr int TS = NPB / TILE_X;
@@ -386,7 +394,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WavesNorm
     const int ox = DIR == 1 ? 0 : 1, oy = DIR == 2 ? 0 : 1, oz = DIR == 3 ? 0 : 1;
     int i, j, k;
     bool active;                            // idle lanes recompute a clamped position, store nothing
-    tile_item<TileAxis<SCHEME, DIR>::value, BLOCK>((int)threadIdx.x, ex, ey, ez, i, j, k, active);
+    tile_item<TileAxis<SCHEME, DIR>::value, BLOCK, TileWidth<SCHEME, 0>::value>((int)threadIdx.x, ex, ey, ez, i, j, k, active);
     i += ox; j += oy; k += oz;
     const long long base = pidx(g, k, j, i), st = axis_stride<DIR>(g);
     const bool lr1 = blockIdx.y == 1;
@@ -467,7 +475,7 @@ __global__ __launch_bounds__(BLOCK) void k3_trans1(const double *__restrict__ NL
     const int side = sc / 5, comp = sc - side * 5;
     int i, j, k;
     bool active;
-    tile_item<TileAxis<SCHEME, T1>::value, BLOCK>((int)threadIdx.x, ex, ey, ez, i, j, k, active);
+    tile_item<TileAxis<SCHEME, T1>::value, BLOCK, TileWidth<SCHEME, 1>::value>((int)threadIdx.x, ex, ey, ez, i, j, k, active);
     i += ox; j += oy; k += oz;
     const long long pos = pidx(g, k, j, i), st = axis_stride<T1>(g);
     const double *src = (side == 0 ? NL : NR) + comp * g.np + pos;
@@ -512,7 +520,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(WavesFlux
     const int a = LPF == 4 ? q >> 1 : q;       // first_lr - 1
     int i, j, k;
     bool active;                               // idle lanes still take part in the exchanges
-    tile_item<T2, BLOCK / LPF>((int)(threadIdx.x / LPF), ex, ey, ez, i, j, k, active);
+    tile_item<FluxAxis<SCHEME, T2>::value, BLOCK / LPF, TileWidth<SCHEME, 2>::value>((int)(threadIdx.x / LPF), ex, ey, ez, i, j, k, active);
     const long long pos = pidx(g, k, j, i), sn = axis_stride<DIR>(g), st = axis_stride<T2>(g);
     const double h = DIR == 1 ? g.dx : (DIR == 2 ? g.dy : g.dz);
     const double ht = T2 == 1 ? g.dx : g.dz;
@@ -822,19 +830,19 @@ static void launch_direction(double *q, const Workspace3 &w, double *F, const Gr
     A.g = g; A.eno = eno; A.ctl = ctl; A.counters = counters;
     if constexpr (SCHEME == SCHEME_SR64) A.mlp.p = (const double *)w.mlp;
     else if constexpr (SCHEME != SCHEME_JS) A.mlp.p = (const float *)w.mlp;
-    const unsigned nbn = tile_blocks<TileAxis<SCHEME, DIR>::value, BLOCK>(DIR == 1 ? g.nx + 2 : g.nx + 4, DIR == 2 ? g.ny + 2 : g.ny + 4,
+    const unsigned nbn = tile_blocks<TileAxis<SCHEME, DIR>::value, BLOCK, TileWidth<SCHEME, 0>::value>(DIR == 1 ? g.nx + 2 : g.nx + 4, DIR == 2 ? g.ny + 2 : g.ny + 4,
                                                  DIR == 3 ? g.nz + 2 : g.nz + 4);
     if (tm) tm->begin(st);
     hipLaunchKernelGGL((k3_normal<DIR, SCHEME>), dim3(nbn, 2), dim3(BLOCK), 0, st, (const double *)q, w.NL, w.NR, A);
     if (tm) tm->end(0, st);
     constexpr int T1 = Trans<DIR>::T1;
-    const unsigned nbt = tile_blocks<TileAxis<SCHEME, T1>::value, BLOCK>(DIR == 1 ? g.nx + 2 : (T1 == 1 ? g.nx : g.nx + 4),
+    const unsigned nbt = tile_blocks<TileAxis<SCHEME, T1>::value, BLOCK, TileWidth<SCHEME, 1>::value>(DIR == 1 ? g.nx + 2 : (T1 == 1 ? g.nx : g.nx + 4),
                                                 DIR == 2 ? g.ny + 2 : g.ny, DIR == 3 ? g.nz + 2 : g.nz + 4);
     if (tm) tm->begin(st);
     hipLaunchKernelGGL((k3_trans1<DIR, SCHEME>), dim3(nbt, 10), dim3(BLOCK), 0, st, (const double *)w.NL,
                        (const double *)w.NR, w.S, A);
     if (tm) tm->end(1, st);
-    const unsigned nbf = tile_blocks<Trans<DIR>::T2, BLOCK / FluxLanes<SCHEME>::value>(
+    const unsigned nbf = tile_blocks<FluxAxis<SCHEME, Trans<DIR>::T2>::value, BLOCK / FluxLanes<SCHEME>::value, TileWidth<SCHEME, 2>::value>(
         DIR == 1 ? g.nx + 1 : g.nx, DIR == 2 ? g.ny + 1 : g.ny, DIR == 3 ? g.nz + 1 : g.nz);
     if (tm) tm->begin(st);
     hipLaunchKernelGGL((k3_flux<DIR, SCHEME, RIEMANN>), dim3(nbf), dim3(BLOCK), 0, st, (const double *)w.S, F, A, dt);
