@@ -149,6 +149,42 @@ __device__ __forceinline__ bool features7(const double q[7], double f[6]) {
     return true;
 }
 
+// features7 for the FP32 closure: the plateau test in FP64, the six inputs in FP32
+// (v_rcp_f32 quotients, v_log_f32 for the scale channel), as the 2-D FP32 closure
+// computes its features; false = plateau (f untouched)
+__device__ __forceinline__ bool features7_32(const double q[7], float f[6]) {
+    double fm[4][3];
+    delta_forms7(q, fm);
+    const double c0 = 1.0 / 36.0, c1 = 13.0 / 12.0, c2 = 781.0 / 720.0;
+    double delta[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) delta[k] = fabs(fm[k][0]) * c0 + fabs(fm[k][1]) * c1 + fabs(fm[k][2]) * c2;
+    double dm = dmax(dmax(dmax(delta[0], delta[1]), delta[2]), delta[3]);
+    double qs = fabs(q[0]);
+#pragma unroll
+    for (int m = 1; m < 7; ++m) qs = dmax(qs, fabs(q[m]));
+    qs = dmax(qs, 1.0);
+    if (dm <= 1.0e-13 * qs) return false;
+    // not a plateau: dm > 1e-13 qs >= 1e-13, above the 1e-15 and 1e-30 floors
+    const float inv = rcp_fast((float)dm);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) f[k] = (float)delta[k] * inv;
+    float g = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double d2 = q[i] - 2.0 * q[i + 1] + q[i + 2];
+        double den = fabs(q[i + 1] - q[i]) + fabs(q[i + 2] - q[i + 1]) + 1.0e-15;
+        float r = (float)fabs(d2) * rcp_fast((float)den);
+        g = i == 0 ? r : fmax_(g, r);
+    }
+    f[4] = fminf(g, 1.0f);
+    const float rel = (float)dm * rcp_fast((float)qs);
+    // (log10(rel) + 16) / 16 = log2(rel) log10(2) / 16 + 1, clipped to [0, 1]
+    const float scale = fmaf(__builtin_amdgcn_logf(rel), (float)(0.30102999566398120 / 16.0), 1.0f);
+    f[5] = fminf(1.0f, fmax_(0.0f, scale));
+    return true;
+}
+
 // ----------------------------------------------------------- the closure
 // W7:301-311: three swish layers, softmax(6 tanh(z / 6)) over 4 outputs; the
 // reflection average 0.5 [M(x) + P4 M(P6 x)] with P6 reversing the deltas.
@@ -403,13 +439,14 @@ __device__ __forceinline__ void closure7_mfma(const float x[6], const Mfma7A &wa
 // wave needs the closure.
 __device__ __forceinline__ bool closure_beta7_wave(const double v[7], bool reversed, const Mfma7A &wa,
                                                    double beta[4], bool &invalid) {
-    double q[7], f[6];
+    double q[7];
 #pragma unroll
     for (int m = 0; m < 7; ++m) q[m] = reversed ? v[6 - m] : v[m];
-    const bool active = features7(q, f);
+    float f[6];
+    const bool active = features7_32(q, f);
     float x[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) x[k] = active ? (float)f[k] : 0.0f;
+    for (int k = 0; k < 6; ++k) x[k] = active ? f[k] : 0.0f;
     double r[4] = {0.0, 0.0, 0.0, 0.0};
     if (__ballot(active) != 0ull) {   // wave-uniform branch: EXEC stays full
         float ra[4], rb[4];
@@ -444,6 +481,33 @@ __device__ __forceinline__ double combine_beta7(const double s[4], const double 
 #pragma unroll
         for (int k = 0; k < 4; ++k) p[k] = w[k] <= 4.0e-7 ? 0.0 : 1.0;
         double ic = 1.0 / dmax(p[0] * w[0] + p[1] * w[1] + p[2] * w[2] + p[3] * w[3], 1.0e-300);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w[k] = p[k] * w[k] * ic;
+    }
+    return w[0] * s[0] + w[1] * s[1] + w[2] * s[2] + w[3] * s[3];
+}
+
+// The FP32 closure's weights: a = d / (beta + 1e-12)^2 with the reciprocal from v_rcp_f64
+// and two Newton steps (within an ulp of the quotient), then combine_beta7's
+// normalisation and ENO cutoff from the same a
+__device__ __forceinline__ void weights_a7_fast(const double d[4], const double beta[4], double a[4]) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double ib = rcp_newton(beta[k] + 1.0e-12);
+        a[k] = d[k] * ib * ib;
+    }
+}
+
+__device__ __forceinline__ double combine_a7_fast(const double s[4], const double a[4], int eno) {
+    double inv = rcp_newton(dmax(a[0] + a[1] + a[2] + a[3], 1.0e-300));
+    double w[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = a[k] * inv;
+    if (eno) {
+        double p[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) p[k] = w[k] <= 4.0e-7 ? 0.0 : 1.0;
+        double ic = rcp_newton(dmax(p[0] * w[0] + p[1] * w[1] + p[2] * w[2] + p[3] * w[3], 1.0e-300));
 #pragma unroll
         for (int k = 0; k < 4; ++k) w[k] = p[k] * w[k] * ic;
     }
@@ -486,18 +550,33 @@ __device__ __forceinline__ double combine_z7(const double s[4], const double d[4
 // the kernels decide per group of four lanes.
 constexpr double Z7_GUARD_RATIO = 50.0;
 
+// z7_guard_trips from the unnormalised weights a of weights_a7_fast (the FP32 closure)
+__device__ __forceinline__ bool z7_guard_trips_a(const double a[4], const double bz[4], double h, bool invalid) {
+    double best = -1.0, bk = bz[0];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const bool up = a[k] > best;
+        best = up ? a[k] : best;
+        bk = up ? bz[k] : bk;
+    }
+    double bmin = dmin(dmin(bz[0], bz[1]), dmin(bz[2], bz[3]));
+    return invalid || bk > Z7_GUARD_RATIO * bmin + 240.0 * h * h * h;
+}
+
 __device__ __forceinline__ bool z7_guard_trips(const double d[4], const double beta[4], const double bz[4],
                                                double h, bool invalid) {
-    int ks = 0;
-    double best = -1.0;
+    // the dominant candidate's indicator is carried by selects (an index into bz would
+    // put bz in scratch memory)
+    double best = -1.0, bk = bz[0];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         double ib = 1.0 / (beta[k] + 1.0e-12);
         double a = d[k] * ib * ib;
-        if (a > best) { best = a; ks = k; }
+        const bool up = a > best;
+        best = up ? a : best;
+        bk = up ? bz[k] : bk;
     }
     double bmin = dmin(dmin(bz[0], bz[1]), dmin(bz[2], bz[3]));
-    double bk = ks == 0 ? bz[0] : (ks == 1 ? bz[1] : (ks == 2 ? bz[2] : bz[3]));
     return invalid || bk > Z7_GUARD_RATIO * bmin + 240.0 * h * h * h;
 }
 
@@ -557,13 +636,15 @@ __device__ __forceinline__ double recon7_normal_mfma(const double q[7], int s, c
     }
     bool invalid;
     closure = closure_beta7_wave(q, s == 1, wa, b, invalid);
+    double a[4];
+    weights_a7_fast(d, b, a);
     trip = false;
     if (z_fallback && closure) {
         double bz[4];
         beta7_sos(q, bz);
-        trip = z7_guard_trips(d, b, bz, h, invalid);
+        trip = z7_guard_trips_a(a, bz, h, invalid);
     }
-    return combine_beta7(sv, d, b, eno);
+    return combine_a7_fast(sv, a, eno);
 }
 
 // recon7_gauss for the FP32 closure on the matrix cores (every lane of the wave calls it)
@@ -577,16 +658,23 @@ __device__ __forceinline__ void recon7_gauss_mfma(const double q[7], const Mfma7
     optimal7<3>(d2);
     bool invalid;
     closure = closure_beta7_wave(q, false, wa, b, invalid);
+    double a1[4], a2[4];   // weights_a7_fast for both points from one reciprocal
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        double ib = rcp_newton(b[k] + 1.0e-12);
+        a1[k] = d1[k] * ib * ib;
+        a2[k] = d2[k] * ib * ib;
+    }
     trip1 = false;
     trip2 = false;
     if (z_fallback && closure) {
         double bz[4];
         beta7_sos(q, bz);
-        trip1 = z7_guard_trips(d1, b, bz, h, invalid);
-        trip2 = z7_guard_trips(d2, b, bz, h, invalid);
+        trip1 = z7_guard_trips_a(a1, bz, h, invalid);
+        trip2 = z7_guard_trips_a(a2, bz, h, invalid);
     }
-    g1 = combine_beta7(s1, d1, b, eno);
-    g2 = combine_beta7(s2, d2, b, eno);
+    g1 = combine_a7_fast(s1, a1, eno);
+    g2 = combine_a7_fast(s2, a2, eno);
 }
 
 // Both transverse Gauss-point values (loca 1 = head 4, loca 2 = head 3) of one
