@@ -92,7 +92,7 @@ SWEEP_TRAFFIC_BYTES_400 = 37.14e6
 # passes; profiles/pmc_traffic_k3_shockbubble_224x88x88.md) and the SR32 L+L~ WENO7
 # sweep launch at 1200x300 (x and y tiles; profiles/pmc_traffic_k7_sweep_xy_doublemach_1200x300.md)
 FLUX3_TRAFFIC_BYTES_224 = 1208.6e6
-SWEEP7_TRAFFIC_BYTES_1200 = 135.3e6
+SWEEP7_TRAFFIC_BYTES_1200 = 139.9e6
 
 
 def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -> float:
