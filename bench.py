@@ -38,7 +38,13 @@ METRIC = "2D Euler cell-updates/s, WENO5-SR vs WENO-JS at 400^2, same L1 error"
 T_END = 0.25
 T_MID = 0.125          # start of the timed window with --steps N (the developed flow)
 NX = NY = 400
-PEAK_VALU_TFLOPS = 78.6     # MI355X FP64 vector / non-packed FP32 vector rate
+# MI355X peaks: FP64 vector 78.6 TFLOP/s; FP32 157.3 TFLOP/s (vector and matrix core: a
+# non-packed v_fma_f32 issues in 2.85 cycles per wave64 against 5.14 for v_fma_f64, and a
+# v_mfma_f32_4x4x1f32 moves 256 multiply-adds in 10.4 cycles, measured).  A kernel's
+# roofline peak is the blend of its algorithmic op mix (blended_peak).
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_VALU_TFLOPS = PEAK_FP64_TFLOPS     # all-FP64 kernels (WENO-JS, the FP64 closure)
 PEAK_HBM_GBS = 8000.0
 REFERENCE_SR32_WALL_S = 14.253   # experiments/14_mixed_precision_timing/tables/weno5_precision_timing.tex:11
 REFERENCE_JS_WALL_S = 6.605      # same table, line 9
@@ -93,6 +99,28 @@ SWEEP_TRAFFIC_BYTES_400 = 37.14e6
 # sweep launch at 1200x300 (x and y tiles; profiles/pmc_traffic_k7_sweep_xy_doublemach_1200x300.md)
 FLUX3_TRAFFIC_BYTES_224 = 1208.6e6
 SWEEP7_TRAFFIC_BYTES_1200 = 139.9e6
+
+
+def blended_peak(total_ops: float, fp32_ops: float) -> float:
+    """Peak TFLOP/s of an op mix: the FP64 part at the FP64 rate, the FP32 part (the FP32
+    closure's features and MLP passes) at the FP32 rate."""
+    if total_ops <= 0.0:
+        return PEAK_FP64_TFLOPS
+    fp32 = min(max(fp32_ops, 0.0), total_ops)
+    return total_ops / ((total_ops - fp32) / PEAK_FP64_TFLOPS + fp32 / PEAK_FP32_TFLOPS)
+
+
+# FP32 operations of one closure evaluation of the FP32 closure (features beyond the FP64
+# sensors, the direct and the reflected 5-10-6-6-3 pass); zero for the FP64 closure
+CLOSURE_FP32 = {"sr32": OPS["features"] + 2 * OPS["mlp_raw"], "srbf16": OPS["features"] + 2 * OPS["mlp_raw"]}
+
+
+def sweep_closures_per_launch(frac_normal: float, frac_gauss: float) -> float:
+    """Closure evaluations of one k_sweep_xy launch (normal heads and Gauss pairs)."""
+    total = 0.0
+    for nfaces, ntrans in ((NX + 1, NY), (NY + 1, NX)):
+        total += nfaces * (ntrans + 4) * 8 * frac_normal + nfaces * ntrans * 8 * frac_gauss
+    return total
 
 
 def sweep_flops_per_launch(scheme: str, frac_normal: float, frac_gauss: float) -> float:
@@ -239,6 +267,9 @@ def bench_shockbubble3d(args) -> int:
     flux_ms = kms[2] / max(1, kn[2])
     flops = flux3_flops_per_launch(args.closure, frac_g)
     achieved = flops / (flux_ms * 1e-3) / 1e12
+    nf3 = ((NX3 + 1) * NY3 * NZ3 + NX3 * (NY3 + 1) * NZ3 + NX3 * NY3 * (NZ3 + 1)) / 3.0
+    fp32_3 = nf3 * 20 * frac_g * CLOSURE_FP32.get(args.closure, 0)
+    peak3 = blended_peak(flops, fp32_3)
     js_flux_ms = kms_js[2] / max(1, kn_js[2])
     js_achieved = flux3_flops_per_launch("js", 0.0) / (js_flux_ms * 1e-3) / 1e12
     per_kernel = {}
@@ -278,13 +309,14 @@ def bench_shockbubble3d(args) -> int:
         "sr_vs_js_wall": wall_sr / wall_js,
         "roofline": {
             "bound": "valu",
-            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
-                       "FP64-vector / non-packed FP32-vector rate",
+            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the op mix's blend "
+                       "of the MI355X FP64 (78.6) and FP32 (157.3 TFLOP/s) vector rates",
             "kernel": f"k3_flux<x|y|z,{args.closure},hllc>",
             "achieved": achieved,
-            "peak": PEAK_VALU_TFLOPS,
+            "peak": peak3,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_VALU_TFLOPS,
+            "frac": achieved / peak3,
+            "peak_fp32_share": fp32_3 / flops,
             "traffic": FLUX3_TRAFFIC_BYTES_224 if args.closure == "sr32" else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": flux_ms,
@@ -461,6 +493,9 @@ def bench_doublemach7(args) -> int:
     both_ms = kms[0] / max(1, kn[0])
     flops = sweep7_flops_per_launch(closure, frac_n, frac_g, True, guard=True)
     achieved = flops / (both_ms * 1e-3) / 1e12
+    closures7 = sweep7_mfma_ops_per_launch(frac_n, frac_g) / (2 * MLP7_MADD_OPS)
+    fp32_7 = closures7 * (OPS7["features"] + 2 * OPS7["mlp_raw"]) if closure == "sr32" else 0.0
+    peak7 = blended_peak(flops, fp32_7)
     js_both_ms = kms_js[0] / max(1, kn_js[0])
     js_achieved = sweep7_flops_per_launch("js", 0.0, 0.0, True) / (js_both_ms * 1e-3) / 1e12
     ncell = NX7 * NY7
@@ -498,16 +533,18 @@ def bench_doublemach7(args) -> int:
             "bound": "valu",
             "compute": ("VALU (FP64 reconstruction, Riemann flux, closure activations) + MFMA "
                         "v_mfma_f32_4x4x1f32 (the FP32 closure's multiply-adds; share of the algorithmic ops "
-                        "in mfma_ops_share); peak is the MI355X FP64-vector / non-packed FP32-vector rate"
+                        "in mfma_ops_share); peak is the op mix's blend of the MI355X FP64 (78.6) and FP32 "
+                        "(157.3 TFLOP/s, vector and matrix core) rates"
                         if closure == "sr32" else
                         "VALU (no MFMA): FP64 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
-                        "FP64-vector / non-packed FP32-vector rate"),
+                        "FP64-vector rate"),
             "mfma_ops_share": (sweep7_mfma_ops_per_launch(frac_n, frac_g) / flops if closure == "sr32" else 0.0),
             "kernel": f"k7_sweep_xy<{closure},hllc,L+L~> (x and y tiles of a right-hand side in one launch)",
             "achieved": achieved,
-            "peak": PEAK_VALU_TFLOPS,
+            "peak": peak7,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_VALU_TFLOPS,
+            "frac": achieved / peak7,
+            "peak_fp32_share": fp32_7 / flops,
             "traffic": SWEEP7_TRAFFIC_BYTES_1200 if closure == "sr32" else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": both_ms,
@@ -619,6 +656,10 @@ def bench_train1d(args) -> int:
     bwd_recon = PLATEAU_RECON_BWD + frac * (SR_RECON_BWD - PLATEAU_RECON_BWD)
     adj_flops = B * N * nsteps * (2 * (recon + ADV_STAGE_FIXED) + 3 * (bwd_recon + 2 * ADV_STAGE_FIXED))
     achieved = adj_flops / (adj_ms * 1e-3) / 1e12
+    # FP32: the MLP pair of every closure stencil in the two recomputed stages, and its
+    # forward recompute and backward in each of the three reverse stages
+    adj_fp32 = B * N * nsteps * frac * (2 * 2 * MLP_RAW + 3 * (2 * MLP_RAW + 2 * 810))
+    peak_adj = blended_peak(adj_flops, adj_fp32)
     cell_steps = B * N * nsteps
     value = cell_steps * iters / wall
     out = {
@@ -643,13 +684,14 @@ def bench_train1d(args) -> int:
         "roofline": {
             "bound": "valu",
             "compute": "VALU: FP32 closure MLP forward/backward + FP64 state; the parameter-gradient sums "
-                       "run as 16x16x4 MFMA outer products; peak is the MI355X FP64-vector / non-packed "
-                       "FP32-vector rate",
+                       "run as 16x16x4 MFMA outer products; peak is the op mix's blend of the MI355X FP64 (78.6) "
+                       "and FP32 (157.3 TFLOP/s) vector rates",
             "kernel": "k_train_adjoint<sr32>",
             "achieved": achieved,
-            "peak": PEAK_VALU_TFLOPS,
+            "peak": peak_adj,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_VALU_TFLOPS,
+            "frac": achieved / peak_adj,
+            "peak_fp32_share": adj_fp32 / adj_flops,
             "traffic": None,
             "avg_launch_ms": adj_ms,
             "launches": iters,
@@ -733,6 +775,8 @@ def bench_gste1d(args) -> int:
     frac = 1.0 - plateau_fraction_1d(u0)
     recon = NORMAL["sr"][0] + frac * (NORMAL["sr"][1] - NORMAL["sr"][0])
     flops_b = 4096 * n * steps * 3 * (recon + ADV_STAGE_FIXED)
+    fp32_b = 4096 * n * steps * 3 * frac * 2 * MLP_RAW if closure == "sr32" else 0.0
+    peak_b = blended_peak(flops_b, fp32_b)
     value = n * steps / (ms_sr * 1e-3)
     out = {
         "metric": "1D advection cell-updates/s, GSTE (N=200, t=10, CFL 0.6), WENO5-SR",
@@ -761,9 +805,9 @@ def bench_gste1d(args) -> int:
                        "figure is that of 4096 copies of the run in one launch",
             "kernel": f"k_adv1d_rollout<{closure}> (batch 4096)",
             "achieved": flops_b / (ms_b * 1e-3) / 1e12,
-            "peak": PEAK_VALU_TFLOPS,
+            "peak": peak_b,
             "unit": "TFLOP/s",
-            "frac": flops_b / (ms_b * 1e-3) / 1e12 / PEAK_VALU_TFLOPS,
+            "frac": flops_b / (ms_b * 1e-3) / 1e12 / peak_b,
             "traffic": None,
             "avg_launch_ms": ms_b,
             "algorithmic_flops_per_launch": flops_b,
@@ -943,6 +987,7 @@ def main() -> int:
     wave_n = c[4] / max(1, c[0])
     wave_g = c[5] / max(1, c[2])
     flops = sweep_flops_per_launch(args.closure, frac_n, frac_g)
+    peak = blended_peak(flops, sweep_closures_per_launch(frac_n, frac_g) * CLOSURE_FP32.get(args.closure, 0))
     avg_ms = sweep_ms / max(1, sweep_n)
     achieved = flops / (avg_ms * 1e-3) / 1e12
     js_avg_ms = js_sweep_ms / max(1, js_sweep_n)
@@ -999,14 +1044,15 @@ def main() -> int:
                                       if args.closure == "sr32" and f_sr is not None else None),
         "roofline": {
             "bound": "valu",
-            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the MI355X "
-                       "FP64-vector / non-packed FP32-vector rate",
+            "compute": "VALU (no MFMA): FP32 closure MLP + FP64 reconstruction/flux; peak is the op mix's blend "
+                       "of the MI355X FP64 (78.6) and FP32 (157.3 TFLOP/s) vector rates",
             "kernel": f"k_sweep_xy<{args.closure},hllc> (x and y tiles of a stage in one launch)",
             "window": window,
             "achieved": achieved,
-            "peak": PEAK_VALU_TFLOPS,
+            "peak": peak,
             "unit": "TFLOP/s",
-            "frac": achieved / PEAK_VALU_TFLOPS,
+            "frac": achieved / peak,
+            "peak_fp32_share": sweep_closures_per_launch(frac_n, frac_g) * CLOSURE_FP32.get(args.closure, 0) / flops,
             "traffic": SWEEP_TRAFFIC_BYTES_400 if args.closure == "sr32" else None,
             "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
             "avg_launch_ms": avg_ms,

@@ -160,3 +160,24 @@ def test_weno7_scalar_is_the_sum_of_its_blocks(blocks7, scheme, gauss):
     assert bench.NORMAL7["sr"][1] == int((b["plateau_test"] + b["candidates_normal"] + b["sr_combine"] + b["features"]
                                           + 2 * b["mlp_raw"] + b["reflection_average"]
                                           + b["beta_from_ratios"]).sum())
+
+
+def test_blended_peak():
+    """The roofline peak of an op mix: FP64 work at 78.6, FP32 work at 157.3 TFLOP/s.
+    An all-FP64 launch (JS) is priced at the FP64 rate, an all-FP32 one at the FP32 rate,
+    and the time at peak of a mix is the sum of its parts' times."""
+    assert bench.blended_peak(1.0e9, 0.0) == bench.PEAK_FP64_TFLOPS
+    assert abs(bench.blended_peak(1.0e9, 1.0e9) - bench.PEAK_FP32_TFLOPS) < 1e-9
+    tot, f32 = 3.0e9, 2.0e9
+    t_peak = (tot - f32) / bench.PEAK_FP64_TFLOPS + f32 / bench.PEAK_FP32_TFLOPS
+    assert abs(bench.blended_peak(tot, f32) - tot / t_peak) < 1e-9
+    # the SR32 sweep's FP32 work is the closure's: features + direct and reflected MLP passes
+    # of every closure evaluation, none at a zero closure fraction
+    assert bench.CLOSURE_FP32["sr32"] == bench.OPS["features"] + 2 * bench.OPS["mlp_raw"]
+    assert bench.sweep_closures_per_launch(0.0, 0.0) == 0.0
+    n = bench.sweep_closures_per_launch(1.0, 1.0)
+    faces_lines = (bench.NX + 1) * (bench.NY + 4) + (bench.NY + 1) * (bench.NX + 4)
+    faces = (bench.NX + 1) * bench.NY + (bench.NY + 1) * bench.NX
+    assert n == 8 * (faces_lines + faces)
+    sr = bench.sweep_flops_per_launch("sr32", 1.0, 1.0)
+    assert bench.PEAK_FP64_TFLOPS < bench.blended_peak(sr, n * bench.CLOSURE_FP32["sr32"]) < bench.PEAK_FP32_TFLOPS

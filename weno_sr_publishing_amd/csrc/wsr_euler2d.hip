@@ -787,7 +787,10 @@ int wsr_euler2d_loop_steps_timed(const wsr_euler2d_config *cfg, double *u, doubl
     SweepTimer timer;
     timer.cap = 3 * 2 * n_steps;
     for (int i = 0; i < timer.cap; ++i)
-        if (hipEventCreate(&timer.ev[i]) != hipSuccess) return WSR_E_HIP;
+        if (hipEventCreate(&timer.ev[i]) != hipSuccess) {
+            while (i-- > 0) (void)hipEventDestroy(timer.ev[i]);
+            return WSR_E_HIP;
+        }
     int32_t step0 = 0, step1 = 0;
     if (!loop_step_count(w.ctl, st, &step0)) rc = WSR_E_HIP;
     const bool own = loop_fills_ghosts(cfg);

@@ -984,7 +984,11 @@ int wsr_euler3_loop_steps_timed(const wsr_euler3_config *cfg, double *q, double 
     Timer3 *tm = new Timer3();
     tm->cap = 27 * n_steps;
     for (int i = 0; i < 2 * tm->cap; ++i)
-        if (hipEventCreate(&tm->ev[i]) != hipSuccess) { delete tm; return WSR_E_HIP; }
+        if (hipEventCreate(&tm->ev[i]) != hipSuccess) {
+            while (i-- > 0) (void)hipEventDestroy(tm->ev[i]);
+            delete tm;
+            return WSR_E_HIP;
+        }
     int32_t step0 = 0, step1 = 0;
     if (!loop_step_count(w.ctl, st, &step0)) rc = WSR_E_HIP;
     if (n_steps > 0) enqueue_params3(cfg, w, st);

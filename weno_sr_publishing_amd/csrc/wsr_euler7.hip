@@ -868,7 +868,11 @@ int wsr_euler7_loop_steps_timed(const wsr_euler7_config *cfg, double *u, double 
     Timer7 *tm = new Timer7();
     tm->cap = 4 * n_steps;
     for (int i = 0; i < 2 * tm->cap; ++i)
-        if (hipEventCreate(&tm->ev[i]) != hipSuccess) { delete tm; return WSR_E_HIP; }
+        if (hipEventCreate(&tm->ev[i]) != hipSuccess) {
+            while (i-- > 0) (void)hipEventDestroy(tm->ev[i]);
+            delete tm;
+            return WSR_E_HIP;
+        }
     int32_t step0 = 0, step1 = 0;
     if (!loop_step_count(w.ctl, st, &step0)) rc = WSR_E_HIP;
     if (n_steps > 0) enqueue_params7(cfg, w, st);
