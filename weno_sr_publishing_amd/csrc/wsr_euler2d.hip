@@ -541,13 +541,16 @@ static void enqueue_params(const wsr_euler2d_config *cfg, const Workspace &w, hi
         pack_neuron_major((const double *)cfg->mlp, dev);
         memcpy(pw.w, dev, sizeof dev);
         nwords = MLP5_COUNT * 2;
+    } else if (scheme == SCHEME_SRBF16) {
+        float ref[MLP5_COUNT];
+        memcpy(ref, cfg->mlp, sizeof ref);
+        pack_bf16_closure(ref, pw.w);   // BF16 weight pairs and FP32 biases for the dot-product closure
+        nwords = BF_WORDS;
     } else {
         float ref[MLP5_COUNT], dev[MLP5_COUNT];
         memcpy(ref, cfg->mlp, sizeof ref);
-        if (scheme == SCHEME_SRBF16)
-            for (int i = 0; i < MLP5_COUNT; ++i) ref[i] = host_round_bf16(ref[i]);
         pack_neuron_major(ref, dev);
-        if (scheme != SCHEME_SRBF16) fold_fp32_closure(dev);   // the FP32 closure evaluates folded parameters
+        fold_fp32_closure(dev);   // the FP32 closure evaluates folded parameters
         memcpy(pw.w, dev, sizeof dev);
         nwords = MLP5_COUNT;
     }

@@ -139,14 +139,4 @@ inline bool loop_step_count(const Ctl *ctl, hipStream_t st, int32_t *step) {
     return true;
 }
 
-inline static float host_round_bf16(float x) {
-    uint32_t u;
-    memcpy(&u, &x, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) return x;
-    u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
-    float y;
-    memcpy(&y, &u, 4);
-    return y;
-}
-
 }  // namespace wsr

@@ -13,6 +13,7 @@
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
 
 namespace wsr {
 
@@ -422,15 +423,6 @@ __device__ __forceinline__ float round_bf16(float x) {
     return __uint_as_float(u);
 }
 
-__device__ __forceinline__ void softmax_head32(const float z[3], float r[3]) {
-    const float sixth = 1.0f / 6.0f;
-    float b0 = 6.0f * tanh_fast(z[0] * sixth), b1 = 6.0f * tanh_fast(z[1] * sixth), b2 = 6.0f * tanh_fast(z[2] * sixth);
-    float bm = fmax_(fmax_(b0, b1), b2);
-    float e0 = exp_fast(b0 - bm), e1 = exp_fast(b1 - bm), e2 = exp_fast(b2 - bm);
-    float inv = rcp_fast(e0 + e1 + e2);
-    r[0] = e0 * inv; r[1] = e1 * inv; r[2] = e2 * inv;
-}
-
 // The FP32 closure on folded parameters (fold_fp32_closure): swish h' = z' / (1 + 2^z'),
 // head e_k = 2^(6L - 12L / (1 + 2^u_k)) = e^(6 tanh(z_k / 6)) in (e^-6, e^6), softmax
 // without the max shift (the bounded logits cannot overflow).
@@ -483,24 +475,107 @@ __device__ __forceinline__ void mlp_pair32_folded(const float x[5], const float 
     softmax_head_folded(zb, rb);
 }
 
-// FP32 (BF16: inputs, weights and activations rounded to bfloat16, FP32 arithmetic)
-template <bool BF16>
-__device__ __forceinline__ void mlp_pair32(const float xa_in[5], const float xb_in[5], const float *w, float ra[3],
-                                           float rb[3]) {
-    float xa[5], xb[5], a1[10], b1[10], a2[6], b2[6], a3[6], b3[6], za[3], zb[3];
+// ------------------------------------------------ BF16 closure on packed BF16 dot products
+// Weights in bfloat16, two per 32-bit word; the layer inputs (the features and the
+// hidden activations) rounded to bfloat16 (round to nearest even) and packed in pairs by
+// v_cvt_pk_bf16_f32; each neuron is its FP32 bias plus v_dot2c_f32_bf16 steps (two
+// exact bf16 x bf16 products accumulated in FP32 per instruction); activations, the
+// bounded head and the softmax in FP32.  Device block, per layer and neuron:
+// [bias (FP32 bits), (w_0, w_1), (w_2, w_3), ...] with the last pair zero-padded
+// for odd K: 4 / 6 / 4 / 4 words per neuron of the 5-10-6-6-3 layers.
+constexpr int BF_L1 = 0, BF_L2 = 40, BF_L3 = 76, BF_L4 = 100, BF_WORDS = 112;
+
+// host: reference-layout FP32 parameters -> the packed BF16 block (weights rounded to
+// nearest even, biases kept in FP32)
+inline void pack_bf16_closure(const float *ref, uint32_t *out) {
+    const int K[4] = {5, 10, 6, 6}, N[4] = {10, 6, 6, 3};
+    const int W[4] = {OFF_W1, OFF_W2, OFF_W3, OFF_W4}, B[4] = {OFF_B1, OFF_B2, OFF_B3, OFF_B4};
+    auto bits = [](float x) { uint32_t u; memcpy(&u, &x, 4); return u; };
+    auto bf = [&](float x) -> uint32_t {
+        uint32_t u = bits(x);
+        if ((u & 0x7f800000u) == 0x7f800000u) return u >> 16;
+        return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+    };
+    int o = 0;
+    for (int l = 0; l < 4; ++l)
+        for (int n = 0; n < N[l]; ++n) {
+            out[o++] = bits(ref[B[l] + n]);
+            for (int k = 0; k < K[l]; k += 2) {
+                uint32_t lo = bf(ref[W[l] + k * N[l] + n]);
+                uint32_t hi = k + 1 < K[l] ? bf(ref[W[l] + (k + 1) * N[l] + n]) : 0u;
+                out[o++] = lo | (hi << 16);
+            }
+        }
+}
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ bf16x2 pk_bf16(float lo, float hi) { return bf16x2{(__bf16)lo, (__bf16)hi}; }
+__device__ __forceinline__ float dot2_bf16(uint32_t w, bf16x2 x, float acc) {
+    return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, w), x, acc, false);
+}
+
+// NP packed input pairs -> N neurons (pre-activation) of one evaluation pair; one
+// scalar base per layer, the neurons' runs at constant offsets from it
+template <int NP, int N, int STRIDE>
+__device__ __forceinline__ void dense_bf16_pair(const bf16x2 *xa, const bf16x2 *xb, const float *wf, int base, float *za,
+                                                float *zb) {
+    kconst_ptr<uint32_t> c = kparams((const uint32_t *)wf + base);
+#pragma unroll
+    for (int o = 0; o < N; ++o) {
+        float a = __builtin_bit_cast(float, c[o * STRIDE]), b = a;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+            a = dot2_bf16(c[o * STRIDE + 1 + k], xa[k], a);
+            b = dot2_bf16(c[o * STRIDE + 1 + k], xb[k], b);
+        }
+        za[o] = a;
+        zb[o] = b;
+    }
+}
+
+// ra = M(xa), rb = M(xb) for the BF16 closure (inputs rounded here).  The FP32 inputs
+// pass through an empty asm so that each is rounded from its FP32 value, as the
+// closure defines it (without it the compiler would fold an FP64 -> FP32 -> BF16
+// chain into one direct rounding).  The head is softmax(6 tanh(z / 6)) in the form
+// e_k = 2^(6L - 12L / (1 + 2^(z_k L / 3))), L = log2 e, without a max shift (the
+// bounded logits cannot overflow).
+__device__ __forceinline__ void mlp_pair_bf16(const float xa_in[5], const float xb_in[5], const float *w, float ra[3],
+                                              float rb[3]) {
+    float xa[5], xb[5];
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
-        xa[k] = BF16 ? round_bf16(xa_in[k]) : xa_in[k];
-        xb[k] = BF16 ? round_bf16(xb_in[k]) : xb_in[k];
+        xa[k] = xa_in[k];
+        xb[k] = xb_in[k];
+        asm volatile("" : "+v"(xa[k]));
+        asm volatile("" : "+v"(xb[k]));
     }
-    auto sw = [](float z) { return BF16 ? round_bf16(swish32(z)) : swish32(z); };
-    auto id = [](float z) { return z; };
-    dense_pair<5, 10>(xa, xb, w, DEV_L1, a1, b1, sw);
-    dense_pair<10, 6>(a1, b1, w, DEV_L2, a2, b2, sw);
-    dense_pair<6, 6>(a2, b2, w, DEV_L3, a3, b3, sw);
-    dense_pair<6, 3>(a3, b3, w, DEV_L4, za, zb, id);
-    softmax_head32(za, ra);
-    softmax_head32(zb, rb);
+    bf16x2 pa[5], pb[5];
+    pa[0] = pk_bf16(xa[0], xa[1]); pa[1] = pk_bf16(xa[2], xa[3]); pa[2] = pk_bf16(xa[4], 0.0f);
+    pb[0] = pk_bf16(xb[0], xb[1]); pb[1] = pk_bf16(xb[2], xb[3]); pb[2] = pk_bf16(xb[4], 0.0f);
+    float za[10], zb[10];
+    dense_bf16_pair<3, 10, 4>(pa, pb, w, BF_L1, za, zb);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        pa[k] = pk_bf16(swish32(za[2 * k]), swish32(za[2 * k + 1]));
+        pb[k] = pk_bf16(swish32(zb[2 * k]), swish32(zb[2 * k + 1]));
+    }
+    dense_bf16_pair<5, 6, 6>(pa, pb, w, BF_L2, za, zb);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        pa[k] = pk_bf16(swish32(za[2 * k]), swish32(za[2 * k + 1]));
+        pb[k] = pk_bf16(swish32(zb[2 * k]), swish32(zb[2 * k + 1]));
+    }
+    dense_bf16_pair<3, 6, 4>(pa, pb, w, BF_L3, za, zb);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        pa[k] = pk_bf16(swish32(za[2 * k]), swish32(za[2 * k + 1]));
+        pb[k] = pk_bf16(swish32(zb[2 * k]), swish32(zb[2 * k + 1]));
+    }
+    dense_bf16_pair<3, 3, 4>(pa, pb, w, BF_L4, za, zb);
+    const float l3 = (float)(LOG2E / 3.0);
+    float ua[3] = {za[0] * l3, za[1] * l3, za[2] * l3}, ub[3] = {zb[0] * l3, zb[1] * l3, zb[2] * l3};
+    softmax_head_folded(ua, ra);
+    softmax_head_folded(ub, rb);
 }
 
 // SR features for the FP32 closure: the plateau rule in FP64 as sr_features (the branch
@@ -559,6 +634,7 @@ __device__ __forceinline__ void sr_ratios32(const float f[5], const MLP &P, doub
 template <int SCHEME, class MLP>
 __device__ __forceinline__ void sr_ratios(const double f[5], const MLP &P, double br[3]) {
     double direct[3], refl[3];
+    static_assert(SCHEME == SCHEME_SR64 || SCHEME == SCHEME_SRBF16, "the FP32 closure runs sr_ratios32");
     if constexpr (SCHEME == SCHEME_SR64) {
         double xr[5] = {f[2], f[1], f[0], f[3], f[4]};
         mlp_pair64(f, xr, P.p, direct, refl);
@@ -566,7 +642,7 @@ __device__ __forceinline__ void sr_ratios(const double f[5], const MLP &P, doubl
         float xd[5] = {(float)f[0], (float)f[1], (float)f[2], (float)f[3], (float)f[4]};
         float xr[5] = {xd[2], xd[1], xd[0], xd[3], xd[4]};
         float rd[3], rr[3];
-        mlp_pair32<SCHEME == SCHEME_SRBF16>(xd, xr, P.p, rd, rr);
+        mlp_pair_bf16(xd, xr, P.p, rd, rr);
         direct[0] = rd[0]; direct[1] = rd[1]; direct[2] = rd[2];
         refl[0] = rr[0]; refl[1] = rr[1]; refl[2] = rr[2];
     }
