@@ -131,14 +131,19 @@ __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
 // LDS plan of one sweep tile (doubles):
 //   FV  [NT1][TN][8]        face values, phase 1 -> phase 2
 //   REG phase 1: T [NN][NT1][4] stencil cells + EIG [N1][4] Roe eigenvalues
-//       phase 2: GV [CH][2][8] Gauss-point values + FL [CH][2][4] fluxes (one chunk)
+//       phase 2: GV [CH][2][8] Gauss-point values + FL [CH][4] face fluxes (one chunk)
+//   PAR [LDS_MLP32 floats]  the FP32 closure's parameters (SR32)
 constexpr int N1 = NT1 * TN;     // face-lines of phase 1 (one per thread)
 constexpr int N2 = TT * TN;      // flux faces of phase 2
 constexpr int CH = SBLOCK / 2;   // phase-2 chunk: one (face, Gauss point) flux per thread
 constexpr int FV_SIZE = NT1 * TN * 8;
 constexpr int P1_SIZE = NN * NT1 * 4 + N1 * 4;
-constexpr int P2_SIZE = CH * 16 + CH * 8;
+constexpr int P2_SIZE = CH * 16 + CH * 4;
 constexpr int REG_SIZE = P1_SIZE > P2_SIZE ? P1_SIZE : P2_SIZE;
+// the FP32 closure's parameters (LDS_MLP32 floats) after FV and REG
+constexpr int PAR_SIZE = LDS_MLP32 / 2;
+constexpr int SMEM_SIZE = FV_SIZE + REG_SIZE + PAR_SIZE;
+static_assert(SMEM_SIZE * 8 <= 32768, "five workgroups per CU: at most 32 KiB of LDS each");
 static_assert(N1 * SPLIT == SBLOCK, "phase 1a assigns one face-line per thread");
 static_assert(N1 % (SBLOCK / 8) == 0, "phase 1b passes cover the face-lines exactly");
 
@@ -153,10 +158,9 @@ template <int SCHEME> struct SweepWaves { static constexpr int value = SCHEME ==
 
 // One 4 x 28 face tile of the DIR sweep (tile coordinates bx along the sweep axis, by
 // across it); smem is the block's LDS region of FV_SIZE + REG_SIZE doubles.
-template <int DIR, int SCHEME, int RIEMANN>
+template <int DIR, int SCHEME, int RIEMANN, class MLPT>
 __device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double *__restrict__ F, const SweepArgs &A,
-                                           const typename MlpSweep<SCHEME>::type &mlp, double dt, double *smem,
-                                           int bx, int by) {
+                                           const MLPT &mlp, double dt, double *smem, int bx, int by) {
     const Grid &g = A.g;
     const double h_n = DIR == 1 ? g.dx : g.dy;   // sweep-axis spacing
     const double h_t = DIR == 1 ? g.dy : g.dx;   // transverse spacing
@@ -262,11 +266,18 @@ __device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double
             double f[4];
             if constexpr (RIEMANN == RIEMANN_FORCE) force_flux<DIR>(ul, ur, dt_h, g.gamma, f);
             else hllc_flux<DIR>(ul, ur, g.gamma, f);
-            double *o = FL + ci * 8 + gp * 4;
-            o[0] = f[0]; o[1] = f[1]; o[2] = f[2]; o[3] = f[3];
+            // the face's two Gauss-point fluxes sit in lanes 2 ci, 2 ci + 1 (both active)
+            double f2[4];
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) f2[kk] = __shfl_xor(f[kk], 1);
+            if (gp == 0) {
+                double *o = FL + ci * 4;
+#pragma unroll
+                for (int kk = 0; kk < 4; ++kk) o[kk] = f[kk] * dt_h + f2[kk] * dt_h;
+            }
         }
         __syncthreads();
-        // 2c: (F(g1) + F(g2)) dt/h, coalesced along the padded row
+        // 2c: the face sums (F(g1) + F(g2)) dt/h out, coalesced along the padded row
         for (int e = threadIdx.x; e < CH * 4; e += SBLOCK) {
             const int ci = e >> 2, kk = e & 3;
             const int it = c0 + ci;
@@ -277,7 +288,7 @@ __device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double
             if (face < nfaces && tt < ntrans) {
                 size_t o = DIR == 1 ? ((size_t)tt * (g.nx + 1) + face) * 4 + kk
                                     : ((size_t)face * g.nx + tt) * 4 + kk;
-                F[o] = FL[ci * 8 + kk] * dt_h + FL[ci * 8 + 4 + kk] * dt_h;
+                F[o] = FL[ci * 4 + kk];
             }
         }
         __syncthreads();
@@ -298,15 +309,27 @@ __global__ __launch_bounds__(SBLOCK) __attribute__((amdgpu_waves_per_eu(SweepWav
         if (A.ctl->done) return;
         dt = A.ctl->dt;
     }
-    __shared__ double smem[FV_SIZE + REG_SIZE];
+    __shared__ __align__(16) double smem[SMEM_SIZE];
     const int gx = (A.g.nx + 1 + TN - 1) / TN, tx = (A.g.ny + TT - 1) / TT;
     const int gy = (A.g.ny + 1 + TN - 1) / TN;
     const int id = blockIdx.x;
-    if (id < gx * tx) {
-        sweep_tile<1, SCHEME, RIEMANN>(bc, Fx, A, mlp, dt, smem, id % gx, id / gx);
+    const int dir = id < gx * tx ? 1 : 2;
+    const int it = dir == 1 ? id : id - gx * tx;
+    const int bx = dir == 1 ? it % gx : it % gy, by = dir == 1 ? it / gx : it / gy;
+    if constexpr (SCHEME == SCHEME_SR32) {
+        // the FP32 closure reads its parameters from LDS, every neuron's run as 16-byte
+        // broadcasts (the tile load's barrier orders these stores before the first read)
+        float *spar = reinterpret_cast<float *>(smem + FV_SIZE + REG_SIZE);
+        for (int t = threadIdx.x; t < LDS_MLP32; t += SBLOCK) {
+            const int src = lds_mlp32_source(t);
+            spar[t] = src >= 0 ? mlp.p[src] : 0.0f;
+        }
+        const MlpL ml{(lds_cfloat *)spar};
+        if (dir == 1) sweep_tile<1, SCHEME, RIEMANN>(bc, Fx, A, ml, dt, smem, bx, by);
+        else sweep_tile<2, SCHEME, RIEMANN>(bc, Fy, A, ml, dt, smem, bx, by);
     } else {
-        const int iy = id - gx * tx;
-        sweep_tile<2, SCHEME, RIEMANN>(bc, Fy, A, mlp, dt, smem, iy % gy, iy / gy);
+        if (dir == 1) sweep_tile<1, SCHEME, RIEMANN>(bc, Fx, A, mlp, dt, smem, bx, by);
+        else sweep_tile<2, SCHEME, RIEMANN>(bc, Fy, A, mlp, dt, smem, bx, by);
     }
 }
 
