@@ -93,11 +93,11 @@ BYTES_PER_SWEEP_CELL = 32 + 32               # state tile read + flux write (FP6
 # HBM bytes per SR32 sweep launch at 400^2 (one launch = the x and the y tiles), from
 # the two rocprofv3 counter passes (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000 after the
 # copy-kernel calibration) recorded in profiles/pmc_traffic_k_sweep_xy_c4_400x400.md
-SWEEP_TRAFFIC_BYTES_400 = 37.14e6
+SWEEP_TRAFFIC_BYTES_400 = 37.00e6
 # the same for the SR32 3-D flux launch at 224x88x88 (mean of the x, y, z face
 # passes; profiles/pmc_traffic_k3_shockbubble_224x88x88.md) and the SR32 L+L~ WENO7
 # sweep launch at 1200x300 (x and y tiles; profiles/pmc_traffic_k7_sweep_xy_doublemach_1200x300.md)
-FLUX3_TRAFFIC_BYTES_224 = 1208.6e6
+FLUX3_TRAFFIC_BYTES_224 = 814.1e6
 SWEEP7_TRAFFIC_BYTES_1200 = 139.9e6
 
 
