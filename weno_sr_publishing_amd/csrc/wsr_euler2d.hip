@@ -131,19 +131,19 @@ __device__ __forceinline__ void item_map(int it, int ntau, int &fl, int &tau) {
 // LDS plan of one sweep tile (doubles):
 //   FV  [NT1][TN][8]        face values, phase 1 -> phase 2
 //   REG phase 1: T [NN][NT1][4] stencil cells + EIG [N1][4] Roe eigenvalues
-//       phase 2: GV [CH][2][8] Gauss-point values + FL [CH][4] face fluxes (one chunk)
+//       phase 2: GV [CH][2][8] Gauss-point values (one chunk)
 //   PAR [LDS_MLP32 floats]  the FP32 closure's parameters (SR32)
 constexpr int N1 = NT1 * TN;     // face-lines of phase 1 (one per thread)
 constexpr int N2 = TT * TN;      // flux faces of phase 2
 constexpr int CH = SBLOCK / 2;   // phase-2 chunk: one (face, Gauss point) flux per thread
 constexpr int FV_SIZE = NT1 * TN * 8;
 constexpr int P1_SIZE = NN * NT1 * 4 + N1 * 4;
-constexpr int P2_SIZE = CH * 16 + CH * 4;
+constexpr int P2_SIZE = CH * 16;
 constexpr int REG_SIZE = P1_SIZE > P2_SIZE ? P1_SIZE : P2_SIZE;
 // the FP32 closure's parameters (LDS_MLP32 floats) after FV and REG
 constexpr int PAR_SIZE = LDS_MLP32 / 2;
 constexpr int SMEM_SIZE = FV_SIZE + REG_SIZE + PAR_SIZE;
-static_assert(SMEM_SIZE * 8 <= 32768, "five workgroups per CU: at most 32 KiB of LDS each");
+static_assert(SMEM_SIZE * 8 <= 160 * 1024 / 6, "six workgroups per CU: at most 26.6 KiB of LDS each");
 static_assert(N1 * SPLIT == SBLOCK, "phase 1a assigns one face-line per thread");
 static_assert(N1 % (SBLOCK / 8) == 0, "phase 1b passes cover the face-lines exactly");
 
@@ -174,7 +174,6 @@ __device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double
     double *T = smem + FV_SIZE;               // phase 1
     double *EIG = T + NN * NT1 * 4;
     double *GV = smem + FV_SIZE;              // phase 2 (reuses the phase-1 region)
-    double *FL = GV + CH * 16;
 
     // cooperative tile load, contiguous along the padded row
     for (int e = threadIdx.x; e < NN * NT1; e += SBLOCK) {
@@ -270,28 +269,22 @@ __device__ __forceinline__ void sweep_tile(const double *__restrict__ bc, double
             double f2[4];
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) f2[kk] = __shfl_xor(f[kk], 1);
+            // the lane of the face's first Gauss point stores (F(g1) + F(g2)) dt/h as two
+            // 16-byte halves: consecutive faces of a line (x) or lines of a face (y) in
+            // consecutive lanes
             if (gp == 0) {
-                double *o = FL + ci * 4;
-#pragma unroll
-                for (int kk = 0; kk < 4; ++kk) o[kk] = f[kk] * dt_h + f2[kk] * dt_h;
+                int fl, tl;
+                item_map<DIR>(c0 + ci, TT, fl, tl);
+                const int face = f0 + fl, tt = t0 + tl;
+                if (face < nfaces && tt < ntrans) {
+                    const size_t o = DIR == 1 ? ((size_t)tt * (g.nx + 1) + face) * 4 : ((size_t)face * g.nx + tt) * 4;
+                    double2 *d2 = reinterpret_cast<double2 *>(F + o);
+                    d2[0] = make_double2(f[0] * dt_h + f2[0] * dt_h, f[1] * dt_h + f2[1] * dt_h);
+                    d2[1] = make_double2(f[2] * dt_h + f2[2] * dt_h, f[3] * dt_h + f2[3] * dt_h);
+                }
             }
         }
-        __syncthreads();
-        // 2c: the face sums (F(g1) + F(g2)) dt/h out, coalesced along the padded row
-        for (int e = threadIdx.x; e < CH * 4; e += SBLOCK) {
-            const int ci = e >> 2, kk = e & 3;
-            const int it = c0 + ci;
-            if (it >= N2) continue;
-            int fl, tl;
-            item_map<DIR>(it, TT, fl, tl);
-            const int face = f0 + fl, tt = t0 + tl;
-            if (face < nfaces && tt < ntrans) {
-                size_t o = DIR == 1 ? ((size_t)tt * (g.nx + 1) + face) * 4 + kk
-                                    : ((size_t)face * g.nx + tt) * 4 + kk;
-                F[o] = FL[ci * 4 + kk];
-            }
-        }
-        __syncthreads();
+        if (c0 + CH < N2) __syncthreads();   // the next chunk reuses GV
     }
     if (A.counters) cnt.flush(A.counters);
 }
